@@ -1,0 +1,16 @@
+"""adfdtd — MI355X-native drop-in for the 3-D staggered-grid acoustic FDTD
+(`run_fdtd!`) of Nano560/acoustic-disguising.
+
+The time stepping runs in libadfdtd.so (hand-written HIP for gfx950); this
+package is the host-side mirror of the reference's Julia API.
+"""
+from .simulation import (CHANNELS, Cpml, Domain, Transceiver, TransceiverGrid, compute_cpml_coefficients,
+                         coords, initial_field, make_cpml, make_domain, make_transceiver, p_rec, p_src,
+                         reset_cpml_memory, run_fdtd, t, transceivers_to_grid, trilinear_interpolation, vn_rec,
+                         vn_src, x, xv, y, yv, z, zv)
+from .geometry import build_scatterer, build_update_mask, outside_predicate
+from .illumination import c_for_n, fibonacci_sphere, illumination_points, impulsive_wavelet, ricker_wavelet
+from .extrapolation import PFromPin, PVFromPin, PVFromPV
+from .solver import Solver, dtype_code
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,108 @@
+"""ctypes binding of libadfdtd.so (include/adfdtd.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when the shared object has not been built, and every call that needs a GPU
+raises when the HIP runtime reports none.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libadfdtd.so")
+
+ADF_F32, ADF_F64, ADF_F32_REF = 1, 2, 3
+ADF_REC_P, ADF_REC_VX, ADF_REC_VY, ADF_REC_VZ, ADF_VN_REC = 0, 1, 2, 3, 4
+ADF_SRC_P, ADF_SRC_VX, ADF_SRC_VY, ADF_SRC_VZ, ADF_VN_SRC = 5, 6, 7, 8, 9
+ADF_GF_NONE, ADF_GF_PV_FROM_PIN, ADF_GF_PV_FROM_PV, ADF_GF_P_FROM_PIN = 0, 1, 2, 3
+
+# Every symbol the public headers declare (checked by the CPU test suite).
+EXPORTS = (
+    "adf_version", "adf_last_error", "adf_ctx_create", "adf_ctx_destroy", "adf_ctx_synchronize",
+    "adf_solver_create", "adf_solver_destroy", "adf_solver_set_field", "adf_solver_get_field",
+    "adf_solver_reset", "adf_solver_set_cpml_memory", "adf_solver_get_cpml_memory",
+    "adf_solver_set_mask", "adf_solver_build_mask", "adf_solver_get_mask",
+    "adf_solver_set_transceivers", "adf_solver_get_trace", "adf_solver_set_trace",
+    "adf_solver_set_gf_map", "adf_solver_run", "adf_solver_get_pressure_slice",
+    "adf_solver_last_step_ms",
+)
+
+
+class AdfDomain(C.Structure):
+    """include/adfdtd.h adf_domain == reference Domain (src/simulation.jl:25-44)."""
+
+    _fields_ = [
+        ("tmax", C.c_double), ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double),
+        ("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64),
+        ("c0", C.c_double), ("r0", C.c_double), ("cf", C.c_double),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("fact_m0", C.c_double), ("fact_m1", C.c_double), ("dt", C.c_double),
+        ("nt", C.c_int64), ("z0", C.c_double),
+    ]
+
+
+class AdfCpml(C.Structure):
+    _fields_ = [("npml", C.c_int64)] + [(n, C.c_void_p) for n in
+                                        ("a_l", "a_r", "a_hl", "a_hr", "b_l", "b_r", "b_hl", "b_hr")]
+
+
+class AdfError(RuntimeError):
+    pass
+
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load():
+    """Load libadfdtd.so; raise if it is missing (no fallback exists)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise AdfError(f"{LIB_PATH} is not built; run `make -C {PKG_DIR}` or __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        p, i64, i32, d = C.c_void_p, C.c_int64, C.c_int, C.c_double
+        sig = {
+            "adf_version": ([], i32),
+            "adf_last_error": ([], C.c_char_p),
+            "adf_ctx_create": ([i32, C.POINTER(p)], i32),
+            "adf_ctx_destroy": ([p], i32),
+            "adf_ctx_synchronize": ([p], i32),
+            "adf_solver_create": ([p, C.POINTER(AdfDomain), i32, C.POINTER(AdfCpml), C.POINTER(p)], i32),
+            "adf_solver_destroy": ([p], i32),
+            "adf_solver_set_field": ([p, p], i32),
+            "adf_solver_get_field": ([p, p], i32),
+            "adf_solver_reset": ([p], i32),
+            "adf_solver_set_cpml_memory": ([p, C.POINTER(p), C.POINTER(p)], i32),
+            "adf_solver_get_cpml_memory": ([p, C.POINTER(p), C.POINTER(p)], i32),
+            "adf_solver_set_mask": ([p, p], i32),
+            "adf_solver_build_mask": ([p, i32, p, p, p, p], i32),
+            "adf_solver_get_mask": ([p, p], i32),
+            "adf_solver_set_transceivers": ([p, i64, p, p, p, p, p], i32),
+            "adf_solver_get_trace": ([p, i32, p], i32),
+            "adf_solver_set_trace": ([p, i32, p], i32),
+            "adf_solver_set_gf_map": ([p, i32, C.POINTER(p), i64, i64, i64, p, i64, p, i64, i32], i32),
+            "adf_solver_run": ([p, i64, i64], i32),
+            "adf_solver_get_pressure_slice": ([p, i32, i64, p], i32),
+            "adf_solver_last_step_ms": ([p, C.POINTER(d)], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+        return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = load().adf_last_error().decode(errors="replace")
+        raise AdfError(f"{what}: {msg} (code {rc})")
+
+
+def ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)

@@ -1,0 +1,96 @@
+// Internal state of the adfdtd C ABI (not part of the public header).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/adfdtd.h"
+
+namespace adf {
+extern thread_local std::string g_err;
+int fail(int code, const std::string& msg);
+}  // namespace adf
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return adf::fail(ADF_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+struct adf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 256;
+};
+
+struct adf_solver {
+    adf_ctx* ctx = nullptr;
+    adf_domain dom{};
+    int dtype = ADF_F32;
+    size_t es = 4;
+    int64_t p = 0;
+    int64_t n3 = 0;
+    int64_t nwx = 0;
+
+    // field (nx,ny,nz,4) and the second buffer of the fused pass
+    void* field = nullptr;
+    void* field2 = nullptr;
+    // CPML memory, reference layout; sizes in elements
+    void* mv[3] = {nullptr, nullptr, nullptr};
+    void* mp[3] = {nullptr, nullptr, nullptr};
+    int64_t mv_n[3] = {0, 0, 0};
+    int64_t mp_n[3] = {0, 0, 0};
+    // a (p,2), b (p,2), ah (p+1,2), bh (p+1,2) in the storage type
+    void* coef = nullptr;
+    // 4 mask bit-planes
+    uint32_t* mask = nullptr;
+    bool has_mask = false;
+
+    // transceivers (device copies; traces in the reference (nt,ntx) layout)
+    int64_t ntx = 0;
+    int32_t* tx_idx = nullptr;
+    void* tx_w = nullptr;
+    void* tx_src = nullptr;
+    void* tx_rec = nullptr;
+    void* tx_dir = nullptr;
+    void* tx_vn_src = nullptr;
+    void* tx_vn_rec = nullptr;
+    int32_t* order_p = nullptr;
+    int32_t* order_v = nullptr;
+    std::vector<int> lev_p;  // level offsets into order_p (size nlev+1)
+    std::vector<int> lev_v;
+    bool fuse_mirror = true;
+
+    // GF extrapolation map
+    int gf_kind = 0;
+    int gf_wb = 1;
+    int64_t gf_ng = 0, gf_d2 = 0, gf_d3 = 0;
+    void* gf_t[4] = {nullptr, nullptr, nullptr, nullptr};
+    int32_t* gf_irec = nullptr;
+    int32_t* gf_isrc = nullptr;
+    int64_t gf_nr = 0, gf_ns = 0;
+    double* gf_h0 = nullptr;
+    double* gf_h1 = nullptr;
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_step_ms = 0.0;
+};
+
+namespace adf {
+int solver_alloc(adf_solver* s, const adf_cpml* cp);
+void solver_free(adf_solver* s);
+int solver_zero_state(adf_solver* s);
+int solver_set_txs(adf_solver* s, int64_t ntx, const int64_t* idx, const void* w, const void* src,
+                   const void* dir, const void* vn_src);
+void* solver_trace_ptr(adf_solver* s, int which);
+int solver_set_gf(adf_solver* s, int kind, const double* const* tensors, int64_t nt_gf, int64_t dim2,
+                  int64_t dim3, const int64_t* iRec, int64_t n_irec, const int64_t* iSrc, int64_t n_isrc,
+                  int writeback);
+int solver_run(adf_solver* s, int64_t start_step, int64_t end_step);
+int solver_slice(adf_solver* s, int axis, int64_t index, void* out);
+int solver_build_mask_device(adf_solver* s, int shape, const double* params, const double* xc,
+                             const double* yc, const double* zc);
+}  // namespace adf

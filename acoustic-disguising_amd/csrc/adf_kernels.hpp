@@ -1,0 +1,464 @@
+// adfdtd device kernels (gfx950, wave64).
+//
+// Arithmetic modes (template flag EXACT):
+//   EXACT = true   the reference's arithmetic: neighbour differences in the
+//                  storage type T, everything that touches a Float64 domain
+//                  scalar in double, true divisions by dx/dy/dz, no
+//                  contraction (the file is built with -ffp-contract=off),
+//                  stores rounded to T.  T = double gives the Float64
+//                  algorithm, T = float the reference's own Float32 runs
+//                  (src/kernels/updates.jl with Float32 arrays and Float64
+//                  `dx`, `fact_m0`, `fact_m1`, src/simulation.jl:25-44).
+//   EXACT = false  float storage, float arithmetic, reciprocal multiplies
+//                  and explicit FMAs (fast path).
+//
+// All indices are 0-based here; the reference's 1-based ranges are noted.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace adf {
+
+// Scalars and pointers of one solve.  Arrays follow the reference's Julia
+// layouts: field channel c at F + c*n3 with x fastest; CPML memory
+// mv_x (p,ny,nz,2), mv_y (nx,p,nz,2), mv_z (nx,ny,p,2), mp_* with p+1;
+// coefficients a,b (p,2), ah,bh (p+1,2).
+template <typename T>
+struct Solve {
+    T* P; T* Vx; T* Vy; T* Vz;
+    T* mvx; T* mvy; T* mvz;
+    T* mpx; T* mpy; T* mpz;
+    const T* a; const T* b; const T* ah; const T* bh;
+    const uint32_t* mask;  // 4 bit-planes of nz*ny*nwx words, or null
+    int nx, ny, nz, p, nwx;
+    int cpml_p;            // CPML in the P update (p>0 && (mask || p>1))
+    int fuse_mirror;       // write the rigid mirrors inside the V update
+    double dx, dy, dz, fq, fv;
+    float rdx, rdy, rdz, fqf, fvf;
+};
+
+__device__ __forceinline__ bool mask_bit(const uint32_t* m, int plane, int nwx, int ny, int nz,
+                                         int i, int j, int k) {
+    const uint32_t w = m[((size_t)plane * nz * ny + (size_t)k * ny + j) * nwx + (i >> 5)];
+    return (w >> (i & 31)) & 1u;
+}
+
+// derivatives.jl:66-79 (getBoundaryIndex), 0-based: returns side 0 (none),
+// 1 (left, l = i) or 2 (right, l = i - (n - npml)).
+__device__ __forceinline__ int slab(int i, int n, int npml, int& l) {
+    if (i < npml) { l = i; return 1; }
+    l = i - (n - npml);
+    return l >= 0 ? 2 : 0;
+}
+
+// ---------------------------------------------------------------- exact
+// One CPML memory update psi = b*psi + a*dv ; dv += psi (updates.jl:58-59).
+template <typename T>
+__device__ __forceinline__ double psi_exact(T* m, T bc, T ac, double dv) {
+    const T bm = bc * (*m);
+    const T nm = (T)((double)bm + (double)ac * dv);
+    *m = nm;
+    return dv + (double)nm;
+}
+__device__ __forceinline__ float psi_fast(float* m, float bc, float ac, float dv) {
+    const float nm = fmaf(bc, *m, ac * dv);
+    *m = nm;
+    return dv + nm;
+}
+
+// Pressure at cell (i,j,k), i in [0,nx-2], (j,k) != (ny-1,nz-1).
+// forward.jl:66-89; updates.jl:21-142,288-378.
+template <typename T, bool EXACT>
+__device__ __forceinline__ void p_cell(const Solve<T>& S, int i, int j, int k) {
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const int sxy = nx * ny;
+    const int c = i + nx * (j + ny * k);
+    const bool rj = (j == ny - 1), rk = (k == nz - 1);
+    if (rj || rk) {
+        // rigid planes: update_p! with -2V/d at j==ny / k==nz, no CPML, no mask
+        if constexpr (EXACT) {
+            const double dvx = (double)(T)(S.Vx[c + 1] - S.Vx[c]) / S.dx;
+            const double dvy = rj ? (double)((T)(-2) * S.Vy[c]) / S.dy
+                                  : (double)(T)(S.Vy[c + nx] - S.Vy[c]) / S.dy;
+            const double dvz = rk ? (double)((T)(-2) * S.Vz[c]) / S.dz
+                                  : (double)(T)(S.Vz[c + sxy] - S.Vz[c]) / S.dz;
+            S.P[c] = (T)((double)S.P[c] - S.fq * ((dvx + dvy) + dvz));
+        } else {
+            const float dvx = (float)(S.Vx[c + 1] - S.Vx[c]) * S.rdx;
+            const float dvy = rj ? (-2.0f * (float)S.Vy[c]) * S.rdy : (float)(S.Vy[c + nx] - S.Vy[c]) * S.rdy;
+            const float dvz = rk ? (-2.0f * (float)S.Vz[c]) * S.rdz : (float)(S.Vz[c + sxy] - S.Vz[c]) * S.rdz;
+            S.P[c] = (T)fmaf(-S.fqf, (dvx + dvy) + dvz, (float)S.P[c]);
+        }
+        return;
+    }
+    const int p = S.p;
+    if constexpr (EXACT) {
+        double dvx = (double)(T)(S.Vx[c + 1] - S.Vx[c]) / S.dx;
+        double dvy = (double)(T)(S.Vy[c + nx] - S.Vy[c]) / S.dy;
+        double dvz = (double)(T)(S.Vz[c + sxy] - S.Vz[c]) / S.dz;
+        if (S.cpml_p) {
+            int l, sd;
+            if ((sd = slab(i, nx, p, l)))
+                dvx = psi_exact(&S.mvx[l + p * (j + ny * (k + nz * (sd - 1)))], S.b[l + p * (sd - 1)],
+                                S.a[l + p * (sd - 1)], dvx);
+            if ((sd = slab(j, ny, p, l)))
+                dvy = psi_exact(&S.mvy[i + nx * (l + p * (k + nz * (sd - 1)))], S.b[l + p * (sd - 1)],
+                                S.a[l + p * (sd - 1)], dvy);
+            if ((sd = slab(k, nz, p, l)))
+                dvz = psi_exact(&S.mvz[i + nx * (j + ny * (l + p * (sd - 1)))], S.b[l + p * (sd - 1)],
+                                S.a[l + p * (sd - 1)], dvz);
+        }
+        double coeff = S.fq;
+        if (S.mask && !mask_bit(S.mask, 0, S.nwx, ny, nz, i, j, k)) coeff = 0.0;
+        S.P[c] = (T)((double)S.P[c] - coeff * ((dvx + dvy) + dvz));
+    } else {
+        float dvx = (float)(S.Vx[c + 1] - S.Vx[c]) * S.rdx;
+        float dvy = (float)(S.Vy[c + nx] - S.Vy[c]) * S.rdy;
+        float dvz = (float)(S.Vz[c + sxy] - S.Vz[c]) * S.rdz;
+        if (S.cpml_p) {
+            int l, sd;
+            if ((sd = slab(i, nx, p, l)))
+                dvx = psi_fast((float*)&S.mvx[l + p * (j + ny * (k + nz * (sd - 1)))], S.b[l + p * (sd - 1)],
+                               S.a[l + p * (sd - 1)], dvx);
+            if ((sd = slab(j, ny, p, l)))
+                dvy = psi_fast((float*)&S.mvy[i + nx * (l + p * (k + nz * (sd - 1)))], S.b[l + p * (sd - 1)],
+                               S.a[l + p * (sd - 1)], dvy);
+            if ((sd = slab(k, nz, p, l)))
+                dvz = psi_fast((float*)&S.mvz[i + nx * (j + ny * (l + p * (sd - 1)))], S.b[l + p * (sd - 1)],
+                               S.a[l + p * (sd - 1)], dvz);
+        }
+        if (!S.mask || mask_bit(S.mask, 0, S.nwx, ny, nz, i, j, k))
+            S.P[c] = (T)fmaf(-S.fqf, (dvx + dvy) + dvz, (float)S.P[c]);
+    }
+}
+
+// One velocity component at cell (i,j,k); axis 0 needs i>=1, 1 needs j>=1,
+// 2 needs k>=1 (the reference ranges 2:nx, 2:ny, 2:nz).  Returns the new
+// value.  updates.jl:192-275,380-501.
+template <typename T, bool EXACT, int AX>
+__device__ __forceinline__ T v_cell(const Solve<T>& S, int i, int j, int k, int c, T Pc) {
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const int q = S.p + 1;
+    T* V = AX == 0 ? S.Vx : (AX == 1 ? S.Vy : S.Vz);
+    T* mp = AX == 0 ? S.mpx : (AX == 1 ? S.mpy : S.mpz);
+    const int st = AX == 0 ? 1 : (AX == 1 ? nx : nx * ny);
+    const int idx = AX == 0 ? i : (AX == 1 ? j : k);
+    const int n = AX == 0 ? nx : (AX == 1 ? ny : nz);
+    T out;
+    if constexpr (EXACT) {
+        const double h = AX == 0 ? S.dx : (AX == 1 ? S.dy : S.dz);
+        double dp = (double)(T)(Pc - S.P[c - st]) / h;
+        if (q > 1) {
+            int l, sd;
+            if ((sd = slab(idx, n, q, l))) {
+                const int mi = AX == 0 ? l + q * (j + ny * (k + nz * (sd - 1)))
+                             : (AX == 1 ? i + nx * (l + q * (k + nz * (sd - 1)))
+                                        : i + nx * (j + ny * (l + q * (sd - 1))));
+                dp = psi_exact(&mp[mi], S.bh[l + q * (sd - 1)], S.ah[l + q * (sd - 1)], dp);
+            }
+        }
+        double coeff = S.fv;
+        if (S.mask && !mask_bit(S.mask, 1 + AX, S.nwx, ny, nz, i, j, k)) coeff = 0.0;
+        out = (T)((double)V[c] - coeff * dp);
+    } else {
+        const float rh = AX == 0 ? S.rdx : (AX == 1 ? S.rdy : S.rdz);
+        float dp = (float)(Pc - S.P[c - st]) * rh;
+        if (q > 1) {
+            int l, sd;
+            if ((sd = slab(idx, n, q, l))) {
+                const int mi = AX == 0 ? l + q * (j + ny * (k + nz * (sd - 1)))
+                             : (AX == 1 ? i + nx * (l + q * (k + nz * (sd - 1)))
+                                        : i + nx * (j + ny * (l + q * (sd - 1))));
+                dp = psi_fast((float*)&mp[mi], S.bh[l + q * (sd - 1)], S.ah[l + q * (sd - 1)], dp);
+            }
+        }
+        out = V[c];
+        if (!S.mask || mask_bit(S.mask, 1 + AX, S.nwx, ny, nz, i, j, k))
+            out = (T)fmaf(-S.fvf, dp, (float)V[c]);
+    }
+    V[c] = out;
+    return out;
+}
+
+// ------------------------------------------------------------ v1 kernels
+// Block (64,4); each thread walks KZ planes of one (i,j) column.
+
+template <typename T, bool EXACT>
+__global__ __launch_bounds__(256) void k_update_p(Solve<T> S, int kz) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    if (i >= S.nx - 1 || j >= S.ny) return;
+    const int k0 = blockIdx.z * kz;
+    const int k1 = min(k0 + kz, S.nz);
+    for (int k = k0; k < k1; ++k) {
+        if (j == S.ny - 1 && k == S.nz - 1) continue;
+        p_cell<T, EXACT>(S, i, j, k);
+    }
+}
+
+template <typename T, bool EXACT>
+__global__ __launch_bounds__(256) void k_update_v(Solve<T> S, int kz) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int nx = S.nx, ny = S.ny;
+    if (i >= nx || j >= ny) return;
+    const int k0 = blockIdx.z * kz;
+    const int k1 = min(k0 + kz, S.nz);
+    for (int k = k0; k < k1; ++k) {
+        const int c = i + nx * (j + ny * k);
+        const T Pc = S.P[c];
+        if (i >= 1) v_cell<T, EXACT, 0>(S, i, j, k, c, Pc);
+        if (j >= 1) {
+            const T v = v_cell<T, EXACT, 1>(S, i, j, k, c, Pc);
+            if (S.fuse_mirror && j == 1) S.Vy[c - nx] = -v;  // forward.jl:135
+        }
+        if (k >= 1) {
+            const T v = v_cell<T, EXACT, 2>(S, i, j, k, c, Pc);
+            if (S.fuse_mirror && k == 1) S.Vz[c - nx * ny] = -v;  // forward.jl:136
+        }
+    }
+}
+
+// forward.jl:134-137 as a separate pass (used when a velocity transceiver
+// touches the mirrored planes).
+template <typename T>
+__global__ void k_mirror(Solve<T> S) {
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nx * nz) {
+        const int i = t % nx, k = t / nx;
+        S.Vy[i + nx * ny * k] = -S.Vy[i + nx * (1 + ny * k)];
+    }
+    if (t < nx * ny) S.Vz[t] = -S.Vz[t + nx * ny];
+}
+
+// ------------------------------------------------------------ transceivers
+// Packed transceivers of one solve (device copy of transceiversToGrid,
+// simulation.jl:387-431): idx int32 [4][ntx][3] 1-based, w [4][ntx][8],
+// src/rec [4][ntx][nt], dir [ntx][4], vn_src/vn_rec [ntx][nt].
+template <typename T>
+struct Txs {
+    const int32_t* idx;
+    const T* w;
+    T* src;
+    T* rec;
+    const T* dir;
+    T* vn_src;
+    T* vn_rec;
+    int ntx, nt;
+};
+
+// derivatives.jl:30-60 for one transceiver on one channel: corners in the
+// reference order (i fastest), record then inject per corner.
+template <typename T>
+__device__ __forceinline__ void interp_one(T* F, int nx, int ny, int nz, const int32_t* id3,
+                                           const T* w8, T* rec, T src) {
+    const int I = id3[0], J = id3[1], K = id3[2];
+    if (!(I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz)) return;
+    T r = *rec;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int a = q & 1, b = (q >> 1) & 1, cc = q >> 2;
+        T* f = &F[(I - 1 + a) + nx * ((J - 1 + b) + ny * (K - 1 + cc))];
+        const T wq = w8[q];
+        const T fv = *f;
+        r = r + wq * fv;
+        *f = fv + wq * src;
+    }
+    *rec = r;
+}
+
+// Pressure transceivers of one conflict-free level (forward.jl:92).
+template <typename T>
+__global__ void k_tx_p(T* P, int nx, int ny, int nz, Txs<T> X, const int32_t* order, int n, int it0) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int is = order[t];
+    const size_t o = (size_t)is * X.nt + it0;
+    interp_one<T>(P, nx, ny, nz, X.idx + 3 * is, X.w + 8 * is, &X.rec[o], X.src[o]);
+}
+
+// Velocity transceivers of one level (forward.jl:111-115), then v2vn!
+// (forward.jl:120) and, when no GF map is active, vn2v! into it+1
+// (forward.jl:130-132) for the same transceiver.
+template <typename T>
+__global__ void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, Txs<T> X, const int32_t* order,
+                       int n, int it0, int do_vn2v) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int is = order[t];
+    const int ntx = X.ntx, nt = X.nt;
+    const size_t o = (size_t)is * nt + it0;
+    const size_t cs = (size_t)ntx * nt;  // channel stride of src/rec
+    T* Fc[3] = {Vx, Vy, Vz};
+#pragma unroll
+    for (int ch = 1; ch <= 3; ++ch)
+        interp_one<T>(Fc[ch - 1], nx, ny, nz, X.idx + 3 * (is + ntx * ch), X.w + 8 * (is + ntx * ch),
+                      &X.rec[o + cs * ch], X.src[o + cs * ch]);
+    const T* d = X.dir + 4 * is;
+    X.vn_rec[o] = X.rec[o + cs] * d[1] + X.rec[o + 2 * cs] * d[2] + X.rec[o + 3 * cs] * d[3];
+    if (do_vn2v && it0 + 1 < nt) {
+        const T vn = X.vn_src[o + 1];
+#pragma unroll
+        for (int ch = 1; ch <= 3; ++ch) X.src[o + 1 + cs * ch] = X.src[o + 1 + cs * ch] + vn * d[ch];
+    }
+}
+
+// vn2v! for every transceiver (used after the GF extrapolation).
+template <typename T>
+__global__ void k_vn2v(Txs<T> X, int it0) {
+    const int is = blockIdx.x * blockDim.x + threadIdx.x;
+    if (is >= X.ntx || it0 + 1 >= X.nt) return;
+    const size_t o = (size_t)is * X.nt + it0 + 1;
+    const size_t cs = (size_t)X.ntx * X.nt;
+    const T vn = X.vn_src[o];
+    const T* d = X.dir + 4 * is;
+#pragma unroll
+    for (int ch = 1; ch <= 3; ++ch) X.src[o + cs * ch] = X.src[o + cs * ch] + vn * d[ch];
+}
+
+// ------------------------------------------------------ GF extrapolation
+// History of the extrapolation inputs, [n_irec][nt] doubles:
+// PVFromPin / PFromPin: pin = (p_rec - vn_rec*z0)/2 (extrapolation.jl:224,241)
+// PVFromPV: h0 = p_rec, h1 = vn_rec (extrapolation.jl:207-208).
+template <typename T>
+__global__ void k_gf_history(Txs<T> X, const int32_t* iRec, int nr, int kind, double z0,
+                             double* h0, double* h1, int it0) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    const size_t o = (size_t)iRec[r] * X.nt + it0;
+    const double p = (double)X.rec[o], vn = (double)X.vn_rec[o];
+    if (kind == 2) {
+        h0[(size_t)r * X.nt + it0] = p;
+        h1[(size_t)r * X.nt + it0] = vn;
+    } else {
+        h0[(size_t)r * X.nt + it0] = (p - vn * z0) / 2.0;
+    }
+}
+
+// One output column per block: the gather of extrapolation.jl:101-192
+// (sum over inputs r and lags k < Tn of G[k, ., .] * hist[r][it0-k]),
+// written into the source slots at it+1 (it0+1 0-based).
+template <typename T, typename G>
+__global__ __launch_bounds__(256) void k_gf_gather(Txs<T> X, int kind, int writeback, const G* g0,
+                                                   const G* g1, const G* g2, const G* g3, int ng,
+                                                   int d2, int d3, const int32_t* iSrc, int nout,
+                                                   int nin, const double* h0, const double* h1,
+                                                   double z0, int it0) {
+    const int s = blockIdx.x;
+    if (s >= nout) return;
+    const int Tn = min(it0 + 1, ng);
+    double acc0 = 0.0, acc1 = 0.0;
+    for (int r = 0; r < nin; ++r) {
+        size_t base;
+        if (kind == 1) base = (size_t)ng * (s + (size_t)d2 * r);       // (ng, nsrc, nrec)
+        else if (kind == 3) base = (size_t)ng * (r + (size_t)d2 * s);  // (ng, nrec, nsrc)
+        else base = (size_t)ng * (r + (size_t)d2 * s);                 // (ng, nm, nout)
+        const double* hr0 = h0 + (size_t)r * X.nt + it0;
+        const double* hr1 = kind == 2 ? h1 + (size_t)r * X.nt + it0 : nullptr;
+        for (int k = threadIdx.x; k < Tn; k += blockDim.x) {
+            const double x0 = hr0[-k];
+            if (kind == 2) {
+                const double x1 = hr1[-k];
+                acc0 += (double)g1[base + k] * x0 + (double)g0[base + k] * x1;  // pv*p + pp*vn
+                acc1 += (double)g3[base + k] * x0 + (double)g2[base + k] * x1;  // vv*p + vp*vn
+            } else {
+                acc0 += (double)g0[base + k] * x0;
+                acc1 += (double)g1[base + k] * x0;
+            }
+        }
+    }
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = acc0;
+    red[1][threadIdx.x] = acc1;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && writeback && it0 + 1 < X.nt) {
+        const size_t o = (size_t)iSrc[s] * X.nt + it0 + 1;
+        const double a0 = red[0][0], a1 = red[1][0];
+        if (kind == 1 || kind == 2) {
+            // p_src <- vp-derived, vn_src <- pp-derived (extrapolation.jl:165-167,215-216)
+            X.src[o] = (T)a1;
+            X.vn_src[o] = (T)a0;
+        } else {
+            // PFromPin: p_src = pos - neg, vn_src = (pos + neg)/z0 (extrapolation.jl:246-247)
+            X.src[o] = (T)(a0 - a1);
+            X.vn_src[o] = (T)((a0 + a1) / z0);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ mask
+// geometry.jl:86-143 predicates at pressure-grid coordinates, packed into
+// the scatterer bit-plane s (1 = solid); then build_update_mask's 4 planes
+// (geometry.jl:250-280).  shape: 1 sphere, 2 cube, 3 cross, 4 ellipsoid.
+__device__ __forceinline__ bool inside_shape(int shape, const double* prm, double x, double y, double z) {
+    if (shape == 1) { const double r = prm[0]; return !(x * x + y * y + z * z > r * r); }
+    if (shape == 2) {
+        const double r = prm[0];
+        return !(fabs(x) > r / 2 || fabs(y) > r / 2 || fabs(z) > r / 2);
+    }
+    if (shape == 3) {
+        const double l = prm[0], s = l / 2, sl = 2.0 * l / 3;
+        auto inc = [&](double X, double Y, double Z) { return fabs(X) < s && fabs(Y) < s && fabs(Z) < s; };
+        if (!inc(x, y, z)) return false;
+        if (inc(x + sl, y + sl, z) || inc(x - sl, y - sl, z) || inc(x + sl, y - sl, z) || inc(x - sl, y + sl, z)) return false;
+        if (inc(x + sl, y, z + sl) || inc(x - sl, y, z - sl) || inc(x + sl, y, z - sl) || inc(x - sl, y, z + sl)) return false;
+        if (inc(x, y + sl, z + sl) || inc(x, y - sl, z - sl) || inc(x, y + sl, z - sl) || inc(x, y - sl, z + sl)) return false;
+        return true;
+    }
+    if (shape == 4) {
+        const double a = prm[0], b = prm[1], c = prm[2];
+        return !(x * x / (a * a) + y * y / (b * b) + z * z / (c * c) > 1.0);
+    }
+    return false;
+}
+
+// One thread per 32-cell word of the solid plane.
+__global__ void k_mask_solid(uint32_t* solid, int shape, const double* prm, const double* xc,
+                             const double* yc, const double* zc, int nx, int ny, int nz, int nwx) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t total = (size_t)nwx * ny * nz;
+    if (t >= total) return;
+    const int wx = (int)(t % nwx);
+    const int j = (int)((t / nwx) % ny);
+    const int k = (int)(t / ((size_t)nwx * ny));
+    uint32_t bits = 0;
+    for (int b = 0; b < 32; ++b) {
+        const int i = wx * 32 + b;
+        if (i < nx && inside_shape(shape, prm, xc[i], yc[j], zc[k])) bits |= 1u << b;
+    }
+    solid[t] = bits;
+}
+
+// Four update planes from the solid plane (geometry.jl:257-272).
+__global__ void k_mask_planes(uint32_t* planes, const uint32_t* solid, int nx, int ny, int nz, int nwx) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t pl = (size_t)nwx * ny * nz;
+    if (t >= pl) return;
+    const int wx = (int)(t % nwx);
+    const int j = (int)((t / nwx) % ny);
+    const int k = (int)(t / ((size_t)nwx * ny));
+    const uint32_t valid = (wx * 32 + 32 <= nx) ? 0xffffffffu : ((1u << (nx - wx * 32)) - 1u);
+    const uint32_t s = solid[t];
+    const uint32_t fl = ~s & valid;
+    // fluid bit of cell i-1 for each i of this word
+    const uint32_t prevw = wx > 0 ? (~solid[t - 1]) : 0u;
+    const uint32_t flm = (fl << 1) | (wx > 0 ? (prevw >> 31) & 1u : 0u);
+    uint32_t vx = fl & flm;
+    if (wx == 0) vx &= ~1u;  // i == 1 stays closed
+    const uint32_t vy = j > 0 ? (fl & ~solid[t - nwx]) : 0u;
+    const uint32_t vz = k > 0 ? (fl & ~solid[t - (size_t)nwx * ny]) : 0u;
+    planes[t] = fl;
+    planes[pl + t] = vx & valid;
+    planes[2 * pl + t] = vy & valid;
+    planes[3 * pl + t] = vz & valid;
+}
+
+}  // namespace adf

@@ -1,0 +1,185 @@
+/*
+ * adfdtd — MI355X (gfx950) 3-D staggered-grid acoustic FDTD, C ABI.
+ *
+ * A drop-in device backend for the reference's `run_fdtd!` path
+ * (Nano560/acoustic-disguising, Julia).  Every array argument is a HOST
+ * pointer laid out exactly as the reference's Julia arrays (column-major,
+ * 1-based indices where the reference stores indices).  The library copies
+ * in on set_* and out on get_*, and never keeps a host pointer after a call
+ * returns.  Every function returns 0 on success and a negative code on
+ * error; adf_last_error() gives the message (thread-local).
+ *
+ * Element type of the `void*` field/trace buffers follows the solver dtype:
+ * ADF_F64 -> double, ADF_F32 / ADF_F32_REF -> float.
+ */
+#ifndef ADFDTD_H
+#define ADFDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADF_ABI_VERSION 1
+
+/* Error codes */
+#define ADF_OK 0
+#define ADF_EINVAL -1   /* bad argument / shape */
+#define ADF_EDEVICE -2  /* HIP runtime error */
+#define ADF_ENOMEM -3   /* device allocation failed or budget exceeded */
+#define ADF_ESTATE -4   /* call not valid in the current state */
+
+/* Solver element types.
+ *   ADF_F64     double storage, the reference arithmetic in double
+ *               (bit-reproducible against a Float64 run of the algorithm).
+ *   ADF_F32     float storage and float arithmetic (fast path).
+ *   ADF_F32_REF float storage with the reference's own mixed arithmetic
+ *               (Float32 arrays, Float64 domain scalars: src/simulation.jl
+ *               Domain fields are Float64, src/AcousticDisguising.jl:57
+ *               sets Float32 arrays). */
+#define ADF_F32 1
+#define ADF_F64 2
+#define ADF_F32_REF 3
+
+/* The reference `Domain` (src/simulation.jl:25-44): same fields, same order,
+ * so a Julia `Ref(dom)` can be passed directly. */
+typedef struct adf_domain {
+    double tmax, xmax, ymax, zmax;
+    int64_t nx, ny, nz;
+    double c0, r0, cf, dx, dy, dz, fact_m0, fact_m1, dt;
+    int64_t nt;
+    double z0;
+} adf_domain;
+
+/* The reference `CPMLCoefficients` (src/kernels/cpmlcoeffs.jl:12-34) of
+ * `Cpml.c` (src/simulation.jl:200-227), as host doubles.  a_l..b_r have
+ * npml entries, a_hl..b_hr npml+1.  Pointers may be NULL when npml == 0. */
+typedef struct adf_cpml {
+    int64_t npml;
+    const double *a_l, *a_r, *a_hl, *a_hr, *b_l, *b_r, *b_hl, *b_hr;
+} adf_cpml;
+
+typedef struct adf_ctx adf_ctx;
+typedef struct adf_solver adf_solver;
+
+/* ---------------------------------------------------------------- context */
+
+/* Library version (ADF_ABI_VERSION). */
+int adf_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char* adf_last_error(void);
+/* Bind a context to HIP device `device`; owns one HIP stream. */
+int adf_ctx_create(int device, adf_ctx** out);
+int adf_ctx_destroy(adf_ctx* ctx);
+/* Block until all work queued on the context's stream has finished. */
+int adf_ctx_synchronize(adf_ctx* ctx);
+
+/* ----------------------------------------------------------------- solver */
+
+/* Replaces the device-side allocations of `@zeros(nx,ny,nz,4)` (the field),
+ * `Cpml(dom; npml, rcoef, fc)` memory (src/simulation.jl:206-227) and the
+ * CPML coefficient arrays.  Field and CPML memory start at zero. */
+int adf_solver_create(adf_ctx* ctx, const adf_domain* dom, int dtype, const adf_cpml* cpml,
+                      adf_solver** out);
+int adf_solver_destroy(adf_solver* s);
+
+/* field (nx,ny,nz,4): P, Vx, Vy, Vz (src/kernels/forward.jl:35-38). */
+int adf_solver_set_field(adf_solver* s, const void* field);
+int adf_solver_get_field(adf_solver* s, void* field);
+
+/* `field .= 0; resetCPMLmemory!(cpml)` (src/greens.jl:161-162;
+ * src/simulation.jl:235-241); also zeroes the receiver buffers. */
+int adf_solver_reset(adf_solver* s);
+
+/* CPML memory `cpml.p[a]` (extent npml+1 along axis a) and `cpml.v[a]`
+ * (extent npml), each (.,.,.,2) (src/simulation.jl:213-220). */
+int adf_solver_set_cpml_memory(adf_solver* s, const void* const p[3], const void* const v[3]);
+int adf_solver_get_cpml_memory(adf_solver* s, void* const p[3], void* const v[3]);
+
+/* Scatterer update mask `va` (src/geometry.jl:250-280): Bool (nx,ny,nz,4),
+ * channel 1 pressure, 2..4 vx,vy,vz faces; NULL = no mask (`va === nothing`,
+ * src/kernels/forward.jl:77-89,99-107). */
+int adf_solver_set_mask(adf_solver* s, const uint8_t* mask4);
+
+/* Build the same mask on the device from a shape predicate evaluated at the
+ * pressure-grid coordinates (src/geometry.jl:86-143,159-176,250-280).
+ * shape: 0 none, 1 sphere(radius), 2 cube(edge), 3 cross(length),
+ * 4 ellipsoid(a,b,c; outside iff x^2/a^2+y^2/b^2+z^2/c^2 > 1, an addition).
+ * xc/yc/zc: the n coordinates of x(d), y(d), z(d) (src/simulation.jl:89-93). */
+int adf_solver_build_mask(adf_solver* s, int shape, const double* params, const double* xc,
+                          const double* yc, const double* zc);
+/* Copy the device mask back in the reference Bool (nx,ny,nz,4) layout. */
+int adf_solver_get_mask(adf_solver* s, uint8_t* mask4);
+
+/* `transceiversToGrid` output (src/simulation.jl:387-431) for ntx
+ * transceivers, channels c = p,vx,vy,vz:
+ *   idx   int64 [4][ntx][3]   ChannelBuffers.i (3,ntx), 1-based (the
+ *                             reference stores them as Float32 and casts
+ *                             with Int(); pass Int.(i))
+ *   w     T     [4][ntx][8]   ChannelBuffers.w (2,2,2,ntx)
+ *   src   T     [4][ntx][nt]  ChannelBuffers.src (nt,ntx)
+ *   dir   T     [ntx][4]      TransceiverGrid.direction (4,ntx)
+ *   vn_src T    [ntx][nt]     TransceiverGrid.vn_src (nt,ntx); NULL = zeros
+ * Receiver buffers (rec, vn_rec) start at zero. */
+int adf_solver_set_transceivers(adf_solver* s, int64_t ntx, const int64_t* idx, const void* w,
+                                const void* src, const void* dir, const void* vn_src);
+
+/* Trace buffers, each (nt,ntx) column-major in the solver dtype:
+ *   which 0..3  ChannelBuffers.rec of p, vx, vy, vz
+ *   which 4     vn_rec
+ *   which 5..8  ChannelBuffers.src of p, vx, vy, vz (as mutated by vn2v!,
+ *               src/kernels/forward.jl:130-132)
+ *   which 9     vn_src
+ * set_trace writes a buffer (e.g. `inject_inner_sources!`,
+ * diagnostics/_check_inject_kh/fdtd_runners.jl:39-45, writes 5 and 9). */
+#define ADF_REC_P 0
+#define ADF_REC_VX 1
+#define ADF_REC_VY 2
+#define ADF_REC_VZ 3
+#define ADF_VN_REC 4
+#define ADF_SRC_P 5
+#define ADF_SRC_VX 6
+#define ADF_SRC_VY 7
+#define ADF_SRC_VZ 8
+#define ADF_VN_SRC 9
+int adf_solver_get_trace(adf_solver* s, int which, void* out);
+int adf_solver_set_trace(adf_solver* s, int which, const void* in);
+
+/* In-loop Green's-function extrapolation (`gf_map` of run_fdtd!,
+ * src/kernels/extrapolation.jl:36-249).  kind:
+ *   ADF_GF_PV_FROM_PIN  tensors = {p_pin, v_pin}, each (nt_gf, nsrc, nrec)
+ *   ADF_GF_PV_FROM_PV   tensors = {pp, pv, vp, vv}, each (nt_gf, nrec_in, nout)
+ *   ADF_GF_P_FROM_PIN   tensors = {pos, neg}, each (nt_gf, nrec, nsrc)
+ * as host doubles in Julia layout.  iRec/iSrc are 1-based transceiver
+ * columns.  writeback 1 adds the extrapolated sources into the source slots
+ * at it+1 (the documented intent, docs/impulsive_gf_hologram_path.md:154-171);
+ * 0 computes and discards them, as the reference's copy semantics do
+ * (extrapolation.jl:215-216,226-232,246-247).  kind 0 clears the map. */
+#define ADF_GF_NONE 0
+#define ADF_GF_PV_FROM_PIN 1
+#define ADF_GF_PV_FROM_PV 2
+#define ADF_GF_P_FROM_PIN 3
+int adf_solver_set_gf_map(adf_solver* s, int kind, const double* const* tensors, int64_t nt_gf,
+                          int64_t dim2, int64_t dim3, const int64_t* iRec, int64_t n_irec,
+                          const int64_t* iSrc, int64_t n_isrc, int writeback);
+
+/* `for it in start_step:end_step; forward_onestep!(...); end`
+ * (src/simulation.jl:542-557; src/kernels/forward.jl:13-139).
+ * Steps are 1-based as in the reference.  Synchronous on return. */
+int adf_solver_run(adf_solver* s, int64_t start_step, int64_t end_step);
+
+/* One pressure slice (the `on_step` snapshots of src/hologram.jl:238-247):
+ * axis 0 -> P[index,:,:] (ny,nz); 1 -> P[:,index,:] (nx,nz);
+ * 2 -> P[:,:,index] (nx,ny); index is 1-based. */
+int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* out);
+
+/* Mean device time per step of the last adf_solver_run (ms), measured
+ * with HIP events on the solver's stream. */
+int adf_solver_last_step_ms(adf_solver* s, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADFDTD_H */

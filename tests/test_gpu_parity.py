@@ -1,0 +1,154 @@
+"""GPU (libadfdtd.so) vs the oracle (CPU restatement of the reference) on
+identical seeded inputs, through the C ABI.
+
+Bars:
+  ADF_F64      vs oracle Float64:  rel-L2 <= 1e-12 (observed bit-exact)
+  ADF_F32_REF  vs oracle r32 (the reference's Float32 arithmetic): <= 1e-6
+  ADF_F32      vs oracle Float64:  rel-L2 <= 1e-5
+on every transceiver trace, the field and the CPML memory.
+"""
+import numpy as np
+import pytest
+
+import adfdtd as A
+import oracle as O
+from cases import build, gpu_run, oracle_solver, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f64": 1e-12, "ref": 1e-6, "fast": 1e-5}
+
+
+def _compare(d, txs, cpml, mask, kind, field=None, gf_pair=None):
+    """kind: 'f64' (GPU F64 vs oracle f64), 'ref' (GPU F32_REF vs oracle r32)
+    or 'fast' (GPU F32 vs oracle f64)."""
+    mode = "r32" if kind == "ref" else "f64"
+    ref = oracle_solver(O, d, txs, cpml, mask, mode, field=field)
+    if gf_pair is not None:
+        ref.set_gf_map(*gf_pair[0])
+    ref.run()
+    dtype = np.float64 if kind == "f64" else np.float32
+    prec = "ref" if kind == "ref" else "fast"
+    f, g, cp = gpu_run(d, txs, cpml, mask, dtype, prec, field=field, gf=None if gf_pair is None else gf_pair[1])
+    tol = TOL[kind]
+    errs = {
+        "p_rec": rel_l2(g.rec[0], ref.tx["rec"][0]),
+        "vx_rec": rel_l2(g.rec[1], ref.tx["rec"][1]),
+        "vy_rec": rel_l2(g.rec[2], ref.tx["rec"][2]),
+        "vz_rec": rel_l2(g.rec[3], ref.tx["rec"][3]),
+        "vn_rec": rel_l2(g.vn_rec, ref.tx["vn_rec"]),
+        "src": rel_l2(g.src, ref.tx["src"]),
+        "P": rel_l2(f[0], ref.field[0]),
+        "V": rel_l2(f[1:], ref.field[1:]),
+    }
+    if cpml.n:
+        errs["psi"] = max(rel_l2(a, b) for a, b in zip(cp.p + cp.v, ref.mp + ref.mv))
+    print(kind, {k: float(f"{v:.3g}") for k, v in errs.items()}, flush=True)
+    bad = {k: v for k, v in errs.items() if not v <= tol}
+    assert not bad, bad
+    return errs
+
+
+@pytest.mark.parametrize("kind", ["f64", "ref", "fast"])
+def test_point_source_cpml(kind):
+    d, txs, cpml, mask = build(n=32, nt=90, npml=6, n_rec=24, rec_radius_cells=8)
+    errs = _compare(d, txs, cpml, mask, kind)
+    if kind in ("f64", "ref"):
+        assert errs["p_rec"] == 0.0 and errs["P"] == 0.0  # same operations, same order
+
+
+@pytest.mark.parametrize("kind", ["f64", "ref", "fast"])
+def test_sphere_mask_velocity_source(kind):
+    d, txs, cpml, mask = build(n=(36, 33, 30), nt=80, npml=5, shape="sphere", shape_params=[5 * 0.00686],
+                               n_rec=30, rec_radius_cells=9, src_cells=(8.3, -1.2, 0.7),
+                               src_dir=(0.0, 0.6, 0.8, 0.0))
+    _compare(d, txs, cpml, mask, kind)
+
+
+@pytest.mark.parametrize("npml,shape", [(0, "none"), (0, "cube"), (1, "none"), (1, "cube"), (2, "cross")])
+def test_cpml_guards(npml, shape):
+    # update_p_cpml! skips CPML for npml == 1 while update_p_cpml_mask! does not
+    # (updates.jl:46-47 vs :333-337); npml == 0 selects update_p!/update_p_mask!.
+    d, txs, cpml, mask = build(n=24, nt=60, npml=npml, shape=shape, shape_params=[8 * 0.00686], n_rec=10,
+                               rec_radius_cells=7, src_cells=(2.5, 1.5, -0.5))
+    _compare(d, txs, cpml, mask, "f64")
+
+
+def test_overlapping_injectors_keep_sequential_order():
+    # several transceivers inject into shared cells: the result depends on the
+    # order of the transceiver loop (derivatives.jl:30-60); levels keep it.
+    d, txs, cpml, mask = build(n=28, nt=60, npml=4, n_rec=0)
+    rng = np.random.default_rng(3)
+    extra = []
+    for k in range(12):
+        pt = (np.array([0.3, -0.2, 0.1]) + rng.uniform(-0.6, 0.6, 3)) * d.dx
+        extra.append((pt, [1.0, 0.3, -0.2, 0.5], rng.standard_normal(d.nt) * 1e-3))
+    d, txs, cpml, mask = build(n=28, nt=60, npml=4, n_rec=0, extra=extra)
+    for kind in ("f64", "ref"):
+        errs = _compare(d, txs, cpml, mask, kind)
+        assert errs["p_rec"] == 0.0
+
+
+def test_velocity_stencils_on_mirror_planes():
+    # a velocity transceiver touching Vy j<=2 / Vz k<=2 forces the separate
+    # mirror pass (forward.jl:134-137 after the transceivers)
+    d, txs, cpml, mask = build(n=20, nt=50, npml=0, n_rec=4, rec_radius_cells=4)
+    lo = np.array([-d.xmax, -d.ymax, -d.zmax]) + np.array([5.2, 0.7, 0.4]) * d.dx
+    extra = [(lo, [0.0, 0.2, 0.7, 0.7], np.sin(np.arange(d.nt) * 0.3) * 1e-4)]
+    d, txs, cpml, mask = build(n=20, nt=50, npml=0, n_rec=4, rec_radius_cells=4, extra=extra)
+    _compare(d, txs, cpml, mask, "f64")
+
+
+def test_plane_wave_initial_field_hologram_like():
+    # initialField plane wave (simulation.jl:468-518) with surfaces as receivers
+    d, txs, cpml, mask = build(n=(40, 24, 24), nt=70, npml=5, shape="cube", shape_params=[6 * 0.00686],
+                               n_rec=16, rec_radius_cells=7, src_dir=(0.0, 0.0, 0.0, 0.0))
+    f0 = A.initial_field(d, 2000.0, 0.12).astype(np.float64) * np.float32(1.5)
+    _compare(d, txs, cpml, mask, "f64", field=f0)
+    _compare(d, txs, cpml, mask, "ref", field=f0.astype(np.float32))
+
+
+def test_device_mask_bit_exact():
+    d, _, _, _ = build(n=(45, 38, 33), nt=5, npml=3)
+    for shape, prm in (("sphere", [9 * d.dx]), ("cube", [11 * d.dx]), ("cross", [18 * d.dx]),
+                       ("ellipsoid", [12 * d.dx, 7 * d.dx, 9 * d.dx])):
+        s = A.Solver(d, 1, None)
+        s.build_mask(shape, prm)
+        m_dev = s.get_mask()
+        rd = O.domain(d.tmax, d.xmax, d.ymax, d.zmax, d.nx, d.ny, d.nz, d.c0, d.r0, d.cf)
+        m_ref = O.build_update_mask(rd, shape, prm)
+        assert np.array_equal(m_dev, m_ref), shape
+        # and the uploaded reference-format mask round-trips
+        s.set_mask(m_ref)
+        assert np.array_equal(s.get_mask(), m_ref)
+
+
+@pytest.mark.parametrize("writeback", [True, False])
+def test_gf_map_pv_from_pin(writeback):
+    # outer surface (receivers) drives inner sources through a random causal GF
+    rng = np.random.default_rng(11)
+    d, txs, cpml, mask = build(n=26, nt=60, npml=4, n_rec=14, rec_radius_cells=7)
+    n_out, n_in = 8, 6
+    iRec = list(range(2, 2 + n_out))            # 1-based transceiver columns
+    iSrc = list(range(2 + n_out, 2 + n_out + n_in))
+    ng = 17
+    pp = rng.standard_normal((n_out, n_in, ng)) * 1e-2  # numpy of Julia (ng, nsrc, nrec)
+    vp = rng.standard_normal((n_out, n_in, ng)) * 1e-2
+    ref_args = ("pv_from_pin", (pp, vp), iRec, iSrc, writeback)
+    gf = A.PVFromPin(pp, vp, iRec, iSrc, writeback)
+    errs = _compare(d, txs, cpml, mask, "f64", gf_pair=(ref_args, gf))
+    assert errs["src"] <= 1e-12
+
+
+def test_solver_reset_and_rerun_identical():
+    d, txs, cpml, mask = build(n=24, nt=40, npml=3, n_rec=6, rec_radius_cells=6)
+    g = A.transceivers_to_grid(d, txs, np.float64)
+    cp = A.make_cpml(d, 3, 1e-4, 2000.0, np.float64)
+    s = A.Solver(d, 2, cp)
+    s.set_transceivers(g)
+    s.run()
+    a = s.get_trace(0)
+    s.reset()
+    s.set_transceivers(g)
+    s.run()
+    assert np.array_equal(a, s.get_trace(0))
