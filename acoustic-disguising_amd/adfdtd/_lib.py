@@ -26,8 +26,9 @@ EXPORTS = (
     "adf_solver_set_mask", "adf_solver_build_mask", "adf_solver_get_mask",
     "adf_solver_set_transceivers", "adf_solver_get_trace", "adf_solver_set_trace",
     "adf_solver_set_gf_map", "adf_solver_run", "adf_solver_get_pressure_slice",
-    "adf_solver_last_step_ms",
+    "adf_solver_last_step_ms", "adf_solver_set_option",
 )
+ADF_OPT_PATH, ADF_OPT_TILE_Y, ADF_OPT_BLOCK_Z = 1, 2, 3
 
 
 class AdfDomain(C.Structure):
@@ -89,6 +90,7 @@ def load():
             "adf_solver_run": ([p, i64, i64], i32),
             "adf_solver_get_pressure_slice": ([p, i32, i64, p], i32),
             "adf_solver_last_step_ms": ([p, C.POINTER(d)], i32),
+            "adf_solver_set_option": ([p, i32, i64], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -234,6 +234,15 @@ class Solver:
         L.check(self.lib.adf_solver_get_pressure_slice(self.h, axis, index, L.ptr(out)), "pressure_slice")
         return out
 
+    def set_option(self, option: int, value: int) -> None:
+        """Kernel path / tiling knob (ADF_OPT_*); results do not depend on it."""
+        L.check(self.lib.adf_solver_set_option(self.h, option, int(value)), "set_option")
+
+    def set_path(self, path: str = "auto", tile_y: int = 0, block_z: int = 0) -> None:
+        self.set_option(L.ADF_OPT_PATH, {"auto": 0, "two-pass": 1, "fused": 2}[path])
+        self.set_option(L.ADF_OPT_TILE_Y, tile_y)
+        self.set_option(L.ADF_OPT_BLOCK_Z, block_z)
+
     def last_step_ms(self) -> float:
         v = C.c_double()
         L.check(self.lib.adf_solver_last_step_ms(self.h, C.byref(v)), "last_step_ms")

@@ -112,7 +112,7 @@ extern "C" int adf_solver_destroy(adf_solver* s) {
 extern "C" int adf_solver_set_field(adf_solver* s, const void* field) {
     if (!s || !field) return fail(ADF_EINVAL, "adf_solver_set_field: null argument");
     HIPCHK(hipSetDevice(s->ctx->device));
-    HIPCHK(hipMemcpyAsync(s->field, field, 4 * s->n3 * s->es, hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(hipMemcpyAsync(cur_field(s), field, 4 * s->n3 * s->es, hipMemcpyHostToDevice, s->ctx->stream));
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     return ADF_OK;
 }
@@ -120,7 +120,7 @@ extern "C" int adf_solver_set_field(adf_solver* s, const void* field) {
 extern "C" int adf_solver_get_field(adf_solver* s, void* field) {
     if (!s || !field) return fail(ADF_EINVAL, "adf_solver_get_field: null argument");
     HIPCHK(hipSetDevice(s->ctx->device));
-    HIPCHK(hipMemcpyAsync(field, s->field, 4 * s->n3 * s->es, hipMemcpyDeviceToHost, s->ctx->stream));
+    HIPCHK(hipMemcpyAsync(field, cur_field(s), 4 * s->n3 * s->es, hipMemcpyDeviceToHost, s->ctx->stream));
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     return ADF_OK;
 }
@@ -138,8 +138,11 @@ extern "C" int adf_solver_set_cpml_memory(adf_solver* s, const void* const p[3],
     if (!s || !p || !v) return fail(ADF_EINVAL, "adf_solver_set_cpml_memory: null argument");
     HIPCHK(hipSetDevice(s->ctx->device));
     for (int a = 0; a < 3; ++a) {
-        if (s->mp_n[a] && p[a]) HIPCHK(hipMemcpyAsync(s->mp[a], p[a], s->mp_n[a] * s->es, hipMemcpyHostToDevice, s->ctx->stream));
-        if (s->mv_n[a] && v[a]) HIPCHK(hipMemcpyAsync(s->mv[a], v[a], s->mv_n[a] * s->es, hipMemcpyHostToDevice, s->ctx->stream));
+        // both buffers: entries the step never touches must agree in each
+        for (int b = 0; b < 2; ++b) {
+            if (s->mp_n[a] && p[a]) HIPCHK(hipMemcpyAsync(s->mpb[b][a], p[a], s->mp_n[a] * s->es, hipMemcpyHostToDevice, s->ctx->stream));
+            if (s->mv_n[a] && v[a]) HIPCHK(hipMemcpyAsync(s->mvb[b][a], v[a], s->mv_n[a] * s->es, hipMemcpyHostToDevice, s->ctx->stream));
+        }
     }
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     return ADF_OK;
@@ -149,8 +152,8 @@ extern "C" int adf_solver_get_cpml_memory(adf_solver* s, void* const p[3], void*
     if (!s || !p || !v) return fail(ADF_EINVAL, "adf_solver_get_cpml_memory: null argument");
     HIPCHK(hipSetDevice(s->ctx->device));
     for (int a = 0; a < 3; ++a) {
-        if (s->mp_n[a] && p[a]) HIPCHK(hipMemcpyAsync(p[a], s->mp[a], s->mp_n[a] * s->es, hipMemcpyDeviceToHost, s->ctx->stream));
-        if (s->mv_n[a] && v[a]) HIPCHK(hipMemcpyAsync(v[a], s->mv[a], s->mv_n[a] * s->es, hipMemcpyDeviceToHost, s->ctx->stream));
+        if (s->mp_n[a] && p[a]) HIPCHK(hipMemcpyAsync(p[a], cur_mp(s, a), s->mp_n[a] * s->es, hipMemcpyDeviceToHost, s->ctx->stream));
+        if (s->mv_n[a] && v[a]) HIPCHK(hipMemcpyAsync(v[a], cur_mv(s, a), s->mv_n[a] * s->es, hipMemcpyDeviceToHost, s->ctx->stream));
     }
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     return ADF_OK;
@@ -265,6 +268,11 @@ extern "C" int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t in
     if (!s || !out) return fail(ADF_EINVAL, "adf_solver_get_pressure_slice: null argument");
     HIPCHK(hipSetDevice(s->ctx->device));
     return solver_slice(s, axis, index, out);
+}
+
+extern "C" int adf_solver_set_option(adf_solver* s, int option, int64_t value) {
+    if (!s) return fail(ADF_EINVAL, "adf_solver_set_option: null solver");
+    return solver_set_option(s, option, value);
 }
 
 extern "C" int adf_solver_last_step_ms(adf_solver* s, double* ms) {

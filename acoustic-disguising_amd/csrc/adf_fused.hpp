@@ -1,0 +1,552 @@
+// Fused single-pass step for gfx950: the pressure update and the three
+// velocity updates of one time step in one z-marching sweep.
+//
+// Buffers are ping-ponged: the sweep reads the old field / CPML memory (0)
+// and writes the new ones (1), so a wave may recompute the pressure of a
+// neighbouring tile's edge cells (its halo) from untouched old values.
+//
+// One wave64 owns a whole x-row (nx <= 64*CPL; lane l holds the CPL
+// consecutive cells l*CPL.. as one vector load/store), TY rows in y held in
+// registers, and marches KB planes in z.  Per plane it loads every row of the
+// tile plus the halo row below (j0-1), computes P1 on all of them, takes the
+// x-neighbours Vx0(i+1) and P1(i-1) from the adjacent lane with wave
+// shuffles, and writes P1, Vx1, Vy1, Vz1 of its own rows.  P1 of the previous
+// plane stays in registers for Vz; a tile's first plane gets it from a
+// recomputed halo plane (k0-1).  No LDS, no barriers: waves are independent.
+//
+// Cells the reference never updates (P at i = nx, the (ny,nz) edge, Vx at
+// i = 1, Vy at j = 1 / Vz at k = 1 except through the mirrors;
+// forward.jl:67-106,134-137) are copied or mirrored so the new buffer is
+// complete.  Velocity cells next to injected pressure cells are recomputed
+// afterwards by k_fixup_v from the old buffers and the injected P1, which
+// keeps the order of the reference step (P update, P injection, V update).
+#pragma once
+#include <type_traits>
+
+#include "adf_kernels.hpp"
+
+namespace adf {
+
+// Kernel arguments of one solve, kept small so the fused kernel does not
+// run out of scalar registers: one base pointer per buffer, offsets derived
+// on the device.  f0/f1: old/new field (P,Vx,Vy,Vz at c*n3); psi0/psi1:
+// old/new CPML memory, the six arrays mv_x,mv_y,mv_z (P phase) then
+// mp_x,mp_y,mp_z (V phase) at 256-byte aligned offsets; coef: a,b (2p each)
+// then ah,bh (2(p+1) each).
+template <typename T>
+struct Fused {
+    const T* f0;
+    T* f1;
+    const T* psi0;
+    T* psi1;
+    const T* coef;
+    const uint32_t* mask;
+    int nx, ny, nz, p, nwx, n3;
+    int cpml_p, fuse_mirror;
+    int tiles_y, tiles_z, ty, kb;  // tiling of this solve
+    int off_mv[3], off_mp[3];      // element offsets of the CPML arrays in psi0/psi1
+    double dx, dy, dz, fq, fv;
+    float rdx, rdy, rdz, fqf, fvf;
+
+    __device__ __forceinline__ const T* P0() const { return f0; }
+    __device__ __forceinline__ const T* V0(int a) const { return f0 + (size_t)(a + 1) * n3; }
+    __device__ __forceinline__ T* P1() const { return f1; }
+    __device__ __forceinline__ T* V1(int a) const { return f1 + (size_t)(a + 1) * n3; }
+    // CPML memory offsets (each array in the reference layout, 256-byte aligned)
+    __device__ __forceinline__ int mv_off(int a) const { return off_mv[a]; }
+    __device__ __forceinline__ int mp_off(int a) const { return off_mp[a]; }
+    __device__ __forceinline__ T ca(int i) const { return coef[i]; }
+    __device__ __forceinline__ T cb(int i) const { return coef[2 * p + i]; }
+    __device__ __forceinline__ T cah(int i) const { return coef[4 * p + i]; }
+    __device__ __forceinline__ T cbh(int i) const { return coef[4 * p + 2 * (p + 1) + i]; }
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void ldv(const T* p, T (&o)[N]) {
+    if constexpr (N == 1) {
+        o[0] = *p;
+    } else {
+        typedef T V __attribute__((ext_vector_type(N)));
+        const V v = *reinterpret_cast<const V*>(p);
+#pragma unroll
+        for (int q = 0; q < N; ++q) o[q] = v[q];
+    }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void stv(T* p, const T (&o)[N]) {
+    if constexpr (N == 1) {
+        *p = o[0];
+    } else {
+        typedef T V __attribute__((ext_vector_type(N)));
+        V v;
+#pragma unroll
+        for (int q = 0; q < N; ++q) v[q] = o[q];
+        *reinterpret_cast<V*>(p) = v;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_down1(T v) { return __shfl_down(v, 1, 64); }
+template <typename T>
+__device__ __forceinline__ T shfl_up1(T v) { return __shfl_up(v, 1, 64); }
+
+// New pressure at (i,j,k) from old values (updates.jl:21-142,288-378).
+// vxn = Vx0(i+1), vyn = Vy0(j+1), vzn = Vz0(k+1) (ignored where unused).
+template <typename T, bool EXACT>
+__device__ __forceinline__ T p_new(const Fused<T>& F, int i, int j, int k, T p0, T vx0, T vxn, T vy0, T vyn,
+                                   T vz0, T vzn, bool fluid, bool own) {
+    const int nx = F.nx, ny = F.ny, nz = F.nz;
+    if (i >= nx - 1) return p0;
+    const bool rj = (j == ny - 1), rk = (k == nz - 1);
+    if (rj && rk) return p0;
+    if (rj || rk) {
+        if constexpr (EXACT) {
+            const double dvx = (double)(T)(vxn - vx0) / F.dx;
+            const double dvy = rj ? (double)((T)(-2) * vy0) / F.dy : (double)(T)(vyn - vy0) / F.dy;
+            const double dvz = rk ? (double)((T)(-2) * vz0) / F.dz : (double)(T)(vzn - vz0) / F.dz;
+            return (T)((double)p0 - F.fq * ((dvx + dvy) + dvz));
+        } else {
+            const float dvx = (float)(vxn - vx0) * F.rdx;
+            const float dvy = rj ? (-2.0f * (float)vy0) * F.rdy : (float)(vyn - vy0) * F.rdy;
+            const float dvz = rk ? (-2.0f * (float)vz0) * F.rdz : (float)(vzn - vz0) * F.rdz;
+            return (T)fmaf(-F.fqf, (dvx + dvy) + dvz, (float)p0);
+        }
+    }
+    const int p = F.p;
+    if constexpr (EXACT) {
+        double dvx = (double)(T)(vxn - vx0) / F.dx;
+        double dvy = (double)(T)(vyn - vy0) / F.dy;
+        double dvz = (double)(T)(vzn - vz0) / F.dz;
+        if (F.cpml_p) {
+            int l, sd;
+            if ((sd = slab(i, nx, p, l))) {
+                const int mi = F.mv_off(0) + l + p * (j + ny * (k + nz * (sd - 1)));
+                T m = F.psi0[mi];
+                dvx = psi_exact(&m, F.cb(l + p * (sd - 1)), F.ca(l + p * (sd - 1)), dvx);
+                if (own) F.psi1[mi] = m;
+            }
+            if ((sd = slab(j, ny, p, l))) {
+                const int mi = F.mv_off(1) + i + nx * (l + p * (k + nz * (sd - 1)));
+                T m = F.psi0[mi];
+                dvy = psi_exact(&m, F.cb(l + p * (sd - 1)), F.ca(l + p * (sd - 1)), dvy);
+                if (own) F.psi1[mi] = m;
+            }
+            if ((sd = slab(k, nz, p, l))) {
+                const int mi = F.mv_off(2) + i + nx * (j + ny * (l + p * (sd - 1)));
+                T m = F.psi0[mi];
+                dvz = psi_exact(&m, F.cb(l + p * (sd - 1)), F.ca(l + p * (sd - 1)), dvz);
+                if (own) F.psi1[mi] = m;
+            }
+        }
+        const double coeff = fluid ? F.fq : 0.0;
+        return (T)((double)p0 - coeff * ((dvx + dvy) + dvz));
+    } else {
+        float dvx = (float)(vxn - vx0) * F.rdx;
+        float dvy = (float)(vyn - vy0) * F.rdy;
+        float dvz = (float)(vzn - vz0) * F.rdz;
+        if (F.cpml_p) {
+            int l, sd;
+            if ((sd = slab(i, nx, p, l))) {
+                const int mi = F.mv_off(0) + l + p * (j + ny * (k + nz * (sd - 1)));
+                float m = (float)F.psi0[mi];
+                dvx = psi_fast(&m, F.cb(l + p * (sd - 1)), F.ca(l + p * (sd - 1)), dvx);
+                if (own) F.psi1[mi] = (T)m;
+            }
+            if ((sd = slab(j, ny, p, l))) {
+                const int mi = F.mv_off(1) + i + nx * (l + p * (k + nz * (sd - 1)));
+                float m = (float)F.psi0[mi];
+                dvy = psi_fast(&m, F.cb(l + p * (sd - 1)), F.ca(l + p * (sd - 1)), dvy);
+                if (own) F.psi1[mi] = (T)m;
+            }
+            if ((sd = slab(k, nz, p, l))) {
+                const int mi = F.mv_off(2) + i + nx * (j + ny * (l + p * (sd - 1)));
+                float m = (float)F.psi0[mi];
+                dvz = psi_fast(&m, F.cb(l + p * (sd - 1)), F.ca(l + p * (sd - 1)), dvz);
+                if (own) F.psi1[mi] = (T)m;
+            }
+        }
+        return fluid ? (T)fmaf(-F.fqf, (dvx + dvy) + dvz, (float)p0) : p0;
+    }
+}
+
+// New velocity component AX at (i,j,k) from V0, the new pressure at the cell
+// (pc) and at its minus neighbour along AX (pm) (updates.jl:192-275,380-501).
+template <typename T, bool EXACT, int AX>
+__device__ __forceinline__ T v_new(const Fused<T>& F, int i, int j, int k, T v0, T pc, T pm, bool open) {
+    const int nx = F.nx, ny = F.ny, nz = F.nz;
+    const int q = F.p + 1;
+    const T* mp0 = F.psi0 + F.mp_off(AX);
+    T* mp1 = F.psi1 + F.mp_off(AX);
+    const int idx = AX == 0 ? i : (AX == 1 ? j : k);
+    const int n = AX == 0 ? nx : (AX == 1 ? ny : nz);
+    if constexpr (EXACT) {
+        const double h = AX == 0 ? F.dx : (AX == 1 ? F.dy : F.dz);
+        double dp = (double)(T)(pc - pm) / h;
+        if (q > 1) {
+            int l, sd;
+            if ((sd = slab(idx, n, q, l))) {
+                const int mi = AX == 0 ? l + q * (j + ny * (k + nz * (sd - 1)))
+                             : (AX == 1 ? i + nx * (l + q * (k + nz * (sd - 1)))
+                                        : i + nx * (j + ny * (l + q * (sd - 1))));
+                T m = mp0[mi];
+                dp = psi_exact(&m, F.cbh(l + q * (sd - 1)), F.cah(l + q * (sd - 1)), dp);
+                mp1[mi] = m;
+            }
+        }
+        const double coeff = open ? F.fv : 0.0;
+        return (T)((double)v0 - coeff * dp);
+    } else {
+        const float rh = AX == 0 ? F.rdx : (AX == 1 ? F.rdy : F.rdz);
+        float dp = (float)(pc - pm) * rh;
+        if (q > 1) {
+            int l, sd;
+            if ((sd = slab(idx, n, q, l))) {
+                const int mi = AX == 0 ? l + q * (j + ny * (k + nz * (sd - 1)))
+                             : (AX == 1 ? i + nx * (l + q * (k + nz * (sd - 1)))
+                                        : i + nx * (j + ny * (l + q * (sd - 1))));
+                float m = (float)mp0[mi];
+                dp = psi_fast(&m, F.cbh(l + q * (sd - 1)), F.cah(l + q * (sd - 1)), dp);
+                mp1[mi] = (T)m;
+            }
+        }
+        return open ? (T)fmaf(-F.fvf, dp, (float)v0) : v0;
+    }
+}
+
+__device__ __forceinline__ uint32_t mask_word(const uint32_t* m, int plane, int nwx, int ny, int nz, int i,
+                                              int j, int k) {
+    return m[((size_t)plane * nz * ny + (size_t)k * ny + j) * nwx + (i >> 5)];
+}
+
+// Arithmetic of the fused sweep.  EXACT: the reference's (T differences,
+// double scalars, true divisions); otherwise float with reciprocals.
+template <typename T, bool EXACT>
+struct FA {
+    typedef typename std::conditional<EXACT, double, float>::type A;
+    static __device__ __forceinline__ A diff(T hi, T lo, double h, float rh) {
+        if constexpr (EXACT) return (double)(T)(hi - lo) / h;
+        else return (float)(hi - lo) * rh;
+    }
+    static __device__ __forceinline__ A rigid(T v, double h, float rh) {
+        if constexpr (EXACT) return (double)((T)(-2) * v) / h;
+        else return (-2.0f * (float)v) * rh;
+    }
+    // psi = b*psi + a*dv (updated in place); returns dv + psi
+    static __device__ __forceinline__ A psi(T& m, T bc, T ac, A dv) {
+        if constexpr (EXACT) {
+            const T bm = bc * m;
+            m = (T)((double)bm + (double)ac * dv);
+            return dv + (double)m;
+        } else {
+            m = (T)fmaf((float)bc, (float)m, (float)ac * dv);
+            return dv + (float)m;
+        }
+    }
+    // x1 = x0 - f*s (f = 0 where the mask closes the cell)
+    static __device__ __forceinline__ T upd(T x0, double f, float ff, bool open, A s) {
+        if constexpr (EXACT) return (T)((double)x0 - (open ? f : 0.0) * s);
+        else return open ? (T)fmaf(-ff, s, (float)x0) : x0;
+    }
+};
+
+// One (y-tile, z-block) of one solve, executed by one wave64.  Every load of
+// a plane (field rows, mask words, CPML memory of the slabs the cells lie in)
+// is issued before any arithmetic, and CPML memory is written back after it,
+// so a plane costs one memory round trip.
+//
+// GEN = false is the interior variant: the caller guarantees that every row
+// j0-1..j0+TY-1 and every plane k0-1..k1-1 of the tile lies outside the y/z
+// CPML slabs of both phases, away from the mirrored (1,2) and rigid (n)
+// planes, so only the x direction keeps special cases.  Loads use clamped
+// indices (always in bounds) instead of guards; values of rows, lanes and
+// planes outside the grid are computed but never stored.
+template <typename T, bool EXACT, int CPL, int TY, bool GEN>
+__device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int tzi) {
+    typedef FA<T, EXACT> M_;
+    typedef typename M_::A A;
+    const int nx = F.nx, ny = F.ny, nz = F.nz, p = F.p, qv = p + 1;
+    const int sxy = nx * ny;
+    const int lane = threadIdx.x & 63;
+    const int i0 = lane * CPL;
+    const bool lane_ok = i0 < nx;
+    const int i0c = lane_ok ? i0 : nx - CPL;  // clamped lane start (loads only)
+    const int j0 = tyi * TY;
+    const int k0 = tzi * F.kb;
+    const int k1 = min(k0 + F.kb, nz);
+    const int ks = k0 > 0 ? k0 - 1 : k0;  // first plane (halo plane when k0 > 0)
+    const uint32_t* Mk = F.mask;
+    const int sh = i0c & 31;
+    const bool cpP = F.cpml_p != 0, cpV = qv > 1;
+    const T* __restrict__ P0 = F.P0();
+    const T* __restrict__ Vx0 = F.V0(0);
+    const T* __restrict__ Vy0 = F.V0(1);
+    const T* __restrict__ Vz0 = F.V0(2);
+    const T* __restrict__ psi0 = F.psi0;
+    T* __restrict__ psi1 = F.psi1;
+    auto crow = [&](int j) { return j < 0 ? 0 : (j >= ny ? ny - 1 : j); };
+
+    // per-lane x-slab data (fixed for the whole sweep)
+    int sxP[CPL], lxP[CPL], sxV[CPL], lxV[CPL];
+    T axP[CPL], bxP[CPL], axV[CPL], bxV[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+        const int i = i0 + q;
+        lxP[q] = lxV[q] = 0;
+        sxP[q] = (cpP && i < nx - 1) ? slab(i, nx, p, lxP[q]) : 0;
+        sxV[q] = (cpV && i >= 1 && i < nx) ? slab(i, nx, qv, lxV[q]) : 0;
+        axP[q] = sxP[q] ? F.ca(lxP[q] + p * (sxP[q] - 1)) : (T)0;
+        bxP[q] = sxP[q] ? F.cb(lxP[q] + p * (sxP[q] - 1)) : (T)0;
+        axV[q] = sxV[q] ? F.cah(lxV[q] + qv * (sxV[q] - 1)) : (T)0;
+        bxV[q] = sxV[q] ? F.cbh(lxV[q] + qv * (sxV[q] - 1)) : (T)0;
+    }
+
+    // rows r = -1..TY-1 are stored at index r+1
+    T vzc[TY + 1][CPL];    // Vz0 at the current plane
+    T p1prev[TY + 1][CPL]; // P1 at the previous plane
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) {
+        ldv<T, CPL>(Vz0 + i0c + nx * crow(j0 + r - 1) + sxy * ks, vzc[r]);
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) p1prev[r][q] = (T)0;
+    }
+
+    for (int k = ks; k < k1; ++k) {
+        const bool owned = k >= k0;
+        const int kc = sxy * k;
+        const bool rk = GEN && (k == nz - 1);
+        // z-slab of this plane (uniform; never for interior tiles)
+        int lzP = 0, lzV = 0, szP = 0, szV = 0;
+        T azP = (T)0, bzP = (T)0, azV = (T)0, bzV = (T)0;
+        if constexpr (GEN) {
+            szP = (cpP && !rk) ? slab(k, nz, p, lzP) : 0;
+            szV = (cpV && k >= 1 && owned) ? slab(k, nz, qv, lzV) : 0;
+            if (szP) { azP = F.ca(lzP + p * (szP - 1)); bzP = F.cb(lzP + p * (szP - 1)); }
+            if (szV) { azV = F.cah(lzV + qv * (szV - 1)); bzV = F.cbh(lzV + qv * (szV - 1)); }
+        }
+
+        T p0[TY + 1][CPL], vx0[TY + 1][CPL], vy0[TY + 2][CPL], vzn[TY + 1][CPL], p1[TY + 1][CPL];
+        T mPx[TY + 1][CPL], mVx[TY + 1][CPL];
+        T mPy[GEN ? TY + 1 : 1][CPL], mPz[GEN ? TY + 1 : 1][CPL];
+        T mVy[GEN ? TY + 1 : 1][CPL], mVz[GEN ? TY + 1 : 1][CPL];
+        uint32_t mp_[TY + 1], mvx_[TY + 1], mvy_[TY + 1], mvz_[TY + 1];
+        int syP[TY + 1], lyP[TY + 1], syV[TY + 1], lyV[TY + 1];
+        const int kn = (k + 1 < nz) ? k + 1 : k;
+
+        // ---- all loads of this plane
+#pragma unroll
+        for (int r = 0; r <= TY + 1; ++r) {
+            const int j = j0 + r - 1;
+            const int c = i0c + nx * crow(j) + kc;
+            ldv<T, CPL>(Vy0 + c, vy0[r]);
+            if (r > TY) continue;
+            ldv<T, CPL>(P0 + c, p0[r]);
+            ldv<T, CPL>(Vx0 + c, vx0[r]);
+            ldv<T, CPL>(Vz0 + i0c + nx * crow(j) + sxy * kn, vzn[r]);
+            mp_[r] = mvx_[r] = mvy_[r] = mvz_[r] = 0xffffffffu;
+            if (Mk) {
+                const int jc = crow(j);
+                mp_[r] = mask_word(Mk, 0, F.nwx, ny, nz, i0c, jc, k) >> sh;
+                if (owned && r > 0) {
+                    mvx_[r] = mask_word(Mk, 1, F.nwx, ny, nz, i0c, jc, k) >> sh;
+                    mvy_[r] = mask_word(Mk, 2, F.nwx, ny, nz, i0c, jc, k) >> sh;
+                    mvz_[r] = mask_word(Mk, 3, F.nwx, ny, nz, i0c, jc, k) >> sh;
+                }
+            }
+            const bool rowv = j >= 0 && j < ny;
+            const bool rj = GEN && (j == ny - 1);
+            const bool ok = lane_ok && rowv;
+            lyP[r] = lyV[r] = 0;
+            syP[r] = syV[r] = 0;
+            if constexpr (GEN) {
+                syP[r] = (cpP && rowv && !rj && !rk) ? slab(j, ny, p, lyP[r]) : 0;
+                syV[r] = (cpV && rowv && j >= 1 && owned && r > 0) ? slab(j, ny, qv, lyV[r]) : 0;
+            }
+            if (cpP && ok && !rj && !rk) {
+#pragma unroll
+                for (int q = 0; q < CPL; ++q)
+                    if (sxP[q]) mPx[r][q] = psi0[F.mv_off(0) + lxP[q] + p * (j + ny * (k + nz * (sxP[q] - 1)))];
+            }
+            if (cpV && ok && owned && r > 0) {
+#pragma unroll
+                for (int q = 0; q < CPL; ++q)
+                    if (sxV[q]) mVx[r][q] = psi0[F.mp_off(0) + lxV[q] + qv * (j + ny * (k + nz * (sxV[q] - 1)))];
+            }
+            if constexpr (GEN) {
+                if (ok && syP[r]) ldv<T, CPL>(psi0 + F.mv_off(1) + i0 + nx * (lyP[r] + p * (k + nz * (syP[r] - 1))), mPy[r]);
+                if (ok && szP && !rj) ldv<T, CPL>(psi0 + F.mv_off(2) + i0 + nx * (j + ny * (lzP + p * (szP - 1))), mPz[r]);
+                if (ok && syV[r]) ldv<T, CPL>(psi0 + F.mp_off(1) + i0 + nx * (lyV[r] + qv * (k + nz * (syV[r] - 1))), mVy[r]);
+                if (ok && owned && r > 0 && szV) ldv<T, CPL>(psi0 + F.mp_off(2) + i0 + nx * (j + ny * (lzV + qv * (szV - 1))), mVz[r]);
+            }
+        }
+
+        // ---- new pressure on rows -1..TY-1
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) {
+            const int j = j0 + r - 1;
+            const T vxl = shfl_down1<T>(vx0[r][0]);  // Vx0 of the next lane's first cell
+            const bool rj = GEN && (j == ny - 1);
+            T ayP = (T)0, byP = (T)0;
+            if constexpr (GEN) {
+                if (syP[r]) { ayP = F.ca(lyP[r] + p * (syP[r] - 1)); byP = F.cb(lyP[r] + p * (syP[r] - 1)); }
+            }
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const int i = i0 + q;
+                const T vxn = q + 1 < CPL ? vx0[r][q + 1] : vxl;
+                T out = p0[r][q];
+                if (i < nx - 1 && !(rj && rk)) {
+                    if (GEN && (rj || rk)) {
+                        // rigid planes: -2V/d at j==ny / k==nz, no CPML, no mask
+                        const A dvx = M_::diff(vxn, vx0[r][q], F.dx, F.rdx);
+                        const A dvy = rj ? M_::rigid(vy0[r][q], F.dy, F.rdy) : M_::diff(vy0[r + 1][q], vy0[r][q], F.dy, F.rdy);
+                        const A dvz = rk ? M_::rigid(vzc[r][q], F.dz, F.rdz) : M_::diff(vzn[r][q], vzc[r][q], F.dz, F.rdz);
+                        out = M_::upd(p0[r][q], F.fq, F.fqf, true, (dvx + dvy) + dvz);
+                    } else {
+                        A dvx = M_::diff(vxn, vx0[r][q], F.dx, F.rdx);
+                        A dvy = M_::diff(vy0[r + 1][q], vy0[r][q], F.dy, F.rdy);
+                        A dvz = M_::diff(vzn[r][q], vzc[r][q], F.dz, F.rdz);
+                        if (sxP[q]) dvx = M_::psi(mPx[r][q], bxP[q], axP[q], dvx);
+                        if constexpr (GEN) {
+                            if (syP[r]) dvy = M_::psi(mPy[r][q], byP, ayP, dvy);
+                            if (szP) dvz = M_::psi(mPz[r][q], bzP, azP, dvz);
+                        }
+                        out = M_::upd(p0[r][q], F.fq, F.fqf, (mp_[r] >> q) & 1u, (dvx + dvy) + dvz);
+                    }
+                }
+                p1[r][q] = out;
+            }
+        }
+
+        if (owned) {
+            // ---- velocities, stores and CPML memory of the owned rows
+#pragma unroll
+            for (int r = 1; r <= TY; ++r) {
+                const int j = j0 + r - 1;
+                const T pml = shfl_up1<T>(p1[r][CPL - 1]);  // P1 of the previous lane's last cell
+                if (!lane_ok || j >= ny) continue;
+                const int c = i0 + nx * j + kc;
+                const bool rj = GEN && (j == ny - 1);
+                if (cpP && !rj && !rk) {
+#pragma unroll
+                    for (int q = 0; q < CPL; ++q)
+                        if (sxP[q]) psi1[F.mv_off(0) + lxP[q] + p * (j + ny * (k + nz * (sxP[q] - 1)))] = mPx[r][q];
+                }
+                T ayV = (T)0, byV = (T)0;
+                if constexpr (GEN) {
+                    if (syP[r] && !rj) stv<T, CPL>(psi1 + F.mv_off(1) + i0 + nx * (lyP[r] + p * (k + nz * (syP[r] - 1))), mPy[r]);
+                    if (szP && !rj) stv<T, CPL>(psi1 + F.mv_off(2) + i0 + nx * (j + ny * (lzP + p * (szP - 1))), mPz[r]);
+                    if (syV[r]) { ayV = F.cah(lyV[r] + qv * (syV[r] - 1)); byV = F.cbh(lyV[r] + qv * (syV[r] - 1)); }
+                }
+                T vx1[CPL], vy1[CPL], vz1[CPL];
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const int i = i0 + q;
+                    const T pm = q > 0 ? p1[r][q - 1] : pml;
+                    vx1[q] = vx0[r][q];
+                    if (i >= 1) {
+                        A dp = M_::diff(p1[r][q], pm, F.dx, F.rdx);
+                        if (sxV[q]) dp = M_::psi(mVx[r][q], bxV[q], axV[q], dp);
+                        vx1[q] = M_::upd(vx0[r][q], F.fv, F.fvf, (mvx_[r] >> q) & 1u, dp);
+                    }
+                    vy1[q] = vy0[r][q];
+                    if (!GEN || j >= 1) {
+                        A dp = M_::diff(p1[r][q], p1[r - 1][q], F.dy, F.rdy);
+                        if constexpr (GEN) {
+                            if (syV[r]) dp = M_::psi(mVy[r][q], byV, ayV, dp);
+                        }
+                        vy1[q] = M_::upd(vy0[r][q], F.fv, F.fvf, (mvy_[r] >> q) & 1u, dp);
+                    }
+                    vz1[q] = vzc[r][q];
+                    if (!GEN || k >= 1) {
+                        A dp = M_::diff(p1[r][q], p1prev[r][q], F.dz, F.rdz);
+                        if constexpr (GEN) {
+                            if (szV) dp = M_::psi(mVz[r][q], bzV, azV, dp);
+                        }
+                        vz1[q] = M_::upd(vzc[r][q], F.fv, F.fvf, (mvz_[r] >> q) & 1u, dp);
+                    }
+                }
+                stv<T, CPL>(F.P1() + c, p1[r]);
+                stv<T, CPL>(F.V1(0) + c, vx1);
+                if (!GEN || j >= 1 || !F.fuse_mirror) stv<T, CPL>(F.V1(1) + c, vy1);
+                if (!GEN || k >= 1 || !F.fuse_mirror) stv<T, CPL>(F.V1(2) + c, vz1);
+                if constexpr (GEN) {
+                    if (F.fuse_mirror && (j == 1 || k == 1)) {
+                        T neg[CPL];
+                        if (j == 1) {  // forward.jl:135
+#pragma unroll
+                            for (int q = 0; q < CPL; ++q) neg[q] = -vy1[q];
+                            stv<T, CPL>(F.V1(1) + c - nx, neg);
+                        }
+                        if (k == 1) {  // forward.jl:136
+#pragma unroll
+                            for (int q = 0; q < CPL; ++q) neg[q] = -vz1[q];
+                            stv<T, CPL>(F.V1(2) + c - sxy, neg);
+                        }
+                    }
+                }
+                if (cpV) {
+#pragma unroll
+                    for (int q = 0; q < CPL; ++q)
+                        if (sxV[q]) psi1[F.mp_off(0) + lxV[q] + qv * (j + ny * (k + nz * (sxV[q] - 1)))] = mVx[r][q];
+                }
+                if constexpr (GEN) {
+                    if (syV[r]) stv<T, CPL>(psi1 + F.mp_off(1) + i0 + nx * (lyV[r] + qv * (k + nz * (syV[r] - 1))), mVy[r]);
+                    if (szV) stv<T, CPL>(psi1 + F.mp_off(2) + i0 + nx * (j + ny * (lzV + qv * (szV - 1))), mVz[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r <= TY; ++r)
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                p1prev[r][q] = p1[r][q];
+                vzc[r][q] = vzn[r][q];
+            }
+    }
+}
+
+// One launch for all tiles of a solve: blocks [0, n_gen) take the general
+// tiles of `tiles` (packed tzi << 16 | tyi; started first, they are the
+// slower ones), the rest the interior range [ty_lo, ty_lo+nty) x [tz_lo, ..).
+template <typename T, bool EXACT, int CPL, int TY>
+__global__ __launch_bounds__(64) void k_fused(Fused<T> F, const uint32_t* tiles, int n_gen, int ty_lo, int nty,
+                                              int tz_lo) {
+    const int b = (int)blockIdx.x;
+    if (b < n_gen) {
+        const uint32_t e = tiles[b];
+        fused_tile<T, EXACT, CPL, TY, true>(F, (int)(e & 0xffffu), (int)(e >> 16));
+    } else {
+        const int t = b - n_gen;
+        fused_tile<T, EXACT, CPL, TY, false>(F, ty_lo + t % nty, tz_lo + t / nty);
+    }
+}
+
+// Velocity cells whose update reads an injected pressure cell (and, for
+// tilings that split rows, seam cells), recomputed from V0 / psi0 and the
+// injected P1 (after k_tx_p).  cells: packed (axis << 30 | cell) entries.
+template <typename T, bool EXACT>
+__global__ void k_fixup_v(Fused<T> F, const uint32_t* cells, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t e = cells[t];
+    const int ax = (int)(e >> 30);
+    const int c = (int)(e & 0x3fffffffu);
+    const int nx = F.nx, ny = F.ny, nz = F.nz;
+    const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    const bool open = F.mask ? ((mask_word(F.mask, 1 + ax, F.nwx, ny, nz, i, j, k) >> (i & 31)) & 1u) : true;
+    const T* P1 = F.P1();
+    if (ax == 0) {
+        F.V1(0)[c] = v_new<T, EXACT, 0>(F, i, j, k, F.V0(0)[c], P1[c], P1[c - 1], open);
+    } else if (ax == 1) {
+        const T v = v_new<T, EXACT, 1>(F, i, j, k, F.V0(1)[c], P1[c], P1[c - nx], open);
+        F.V1(1)[c] = v;
+        if (j == 1 && F.fuse_mirror) F.V1(1)[c - nx] = -v;
+    } else {
+        const T v = v_new<T, EXACT, 2>(F, i, j, k, F.V0(2)[c], P1[c], P1[c - nx * ny], open);
+        F.V1(2)[c] = v;
+        if (k == 1 && F.fuse_mirror) F.V1(2)[c - nx * ny] = -v;
+    }
+}
+
+}  // namespace adf

@@ -35,12 +35,15 @@ struct adf_solver {
     int64_t n3 = 0;
     int64_t nwx = 0;
 
-    // field (nx,ny,nz,4) and the second buffer of the fused pass
-    void* field = nullptr;
-    void* field2 = nullptr;
-    // CPML memory, reference layout; sizes in elements
-    void* mv[3] = {nullptr, nullptr, nullptr};
-    void* mp[3] = {nullptr, nullptr, nullptr};
+    // field (nx,ny,nz,4) and CPML memory (reference layouts), two buffers
+    // each: the fused pass reads buffer `cur` and writes buffer `cur ^ 1`
+    void* fieldb[2] = {nullptr, nullptr};
+    void* mvb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    void* mpb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    void* psib[2] = {nullptr, nullptr};  // one allocation per buffer holding mv* then mp*
+    int64_t psi_off[6] = {0, 0, 0, 0, 0, 0};  // element offsets of mv_x..mp_z in psib
+    int64_t psi_n = 0;
+    int cur = 0;
     int64_t mv_n[3] = {0, 0, 0};
     int64_t mp_n[3] = {0, 0, 0};
     // a (p,2), b (p,2), ah (p+1,2), bh (p+1,2) in the storage type
@@ -63,6 +66,21 @@ struct adf_solver {
     std::vector<int> lev_p;  // level offsets into order_p (size nlev+1)
     std::vector<int> lev_v;
     bool fuse_mirror = true;
+    // velocity cells next to pressure-injection stencils (fused path)
+    uint32_t* fix_cells = nullptr;
+    int n_fix = 0;
+
+    // fused-sweep tiling: interior range [ty_lo, ty_lo+nty) x [tz_lo, tz_lo+ntz)
+    // and the list of the remaining (general) tiles, for tiling (tl_ty, tl_kb)
+    int tl_ty = 0, tl_kb = 0;
+    int ty_lo = 0, nty = 0, tz_lo = 0, ntz = 0;
+    uint32_t* gen_tiles = nullptr;
+    int n_gen = 0;
+
+    // kernel path and tiling (adf_solver_set_option)
+    int path = 0;      // 0 auto, 1 two-pass, 2 fused
+    int tile_y = 0;    // 0 auto
+    int block_z = 0;   // 0 auto
 
     // GF extrapolation map
     int gf_kind = 0;
@@ -91,6 +109,12 @@ int solver_set_gf(adf_solver* s, int kind, const double* const* tensors, int64_t
                   int writeback);
 int solver_run(adf_solver* s, int64_t start_step, int64_t end_step);
 int solver_slice(adf_solver* s, int axis, int64_t index, void* out);
+void* cur_field(adf_solver* s);
+void* cur_mv(adf_solver* s, int a);
+void* cur_mp(adf_solver* s, int a);
+int solver_set_option(adf_solver* s, int option, int64_t value);
+bool fused_supported(const adf_solver* s);
+void pick_tiles(const adf_solver* s, int* ty, int* kb);
 int solver_build_mask_device(adf_solver* s, int shape, const double* params, const double* xc,
                              const double* yc, const double* zc);
 }  // namespace adf

@@ -249,22 +249,32 @@ struct Txs {
 };
 
 // derivatives.jl:30-60 for one transceiver on one channel: corners in the
-// reference order (i fastest), record then inject per corner.
-template <typename T>
-__device__ __forceinline__ void interp_one(T* F, int nx, int ny, int nz, const int32_t* id3,
-                                           const T* w8, T* rec, T src) {
+// reference order (i fastest), record then inject per corner.  The eight
+// corners are distinct cells, so all of them are loaded before any is
+// written (same values as the reference's corner-by-corner loop) and the
+// loads overlap instead of forming a load->store chain.
+struct Corners {
+    int c[8];
+    bool ok;
+};
+
+__device__ __forceinline__ Corners corners_of(const int32_t* id3, int nx, int ny, int nz) {
+    Corners C;
     const int I = id3[0], J = id3[1], K = id3[2];
-    if (!(I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz)) return;
+    C.ok = I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz;
+    const int base = (I - 1) + nx * ((J - 1) + ny * (K - 1));
+#pragma unroll
+    for (int q = 0; q < 8; ++q) C.c[q] = base + (q & 1) + nx * ((q >> 1) & 1) + nx * ny * (q >> 2);
+    return C;
+}
+
+template <typename T>
+__device__ __forceinline__ void interp_apply(T* F, const Corners& C, const T* w8, const T (&f)[8], T* rec, T src) {
     T r = *rec;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int a = q & 1, b = (q >> 1) & 1, cc = q >> 2;
-        T* f = &F[(I - 1 + a) + nx * ((J - 1 + b) + ny * (K - 1 + cc))];
-        const T wq = w8[q];
-        const T fv = *f;
-        r = r + wq * fv;
-        *f = fv + wq * src;
-    }
+    for (int q = 0; q < 8; ++q) r = r + w8[q] * f[q];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) F[C.c[q]] = f[q] + w8[q] * src;
     *rec = r;
 }
 
@@ -274,8 +284,13 @@ __global__ void k_tx_p(T* P, int nx, int ny, int nz, Txs<T> X, const int32_t* or
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int is = order[t];
+    const Corners C = corners_of(X.idx + 3 * is, nx, ny, nz);
+    if (!C.ok) return;
+    T f[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) f[q] = P[C.c[q]];
     const size_t o = (size_t)is * X.nt + it0;
-    interp_one<T>(P, nx, ny, nz, X.idx + 3 * is, X.w + 8 * is, &X.rec[o], X.src[o]);
+    interp_apply<T>(P, C, X.w + 8 * is, f, &X.rec[o], X.src[o]);
 }
 
 // Velocity transceivers of one level (forward.jl:111-115), then v2vn!
@@ -291,10 +306,21 @@ __global__ void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, Txs<T> X, co
     const size_t o = (size_t)is * nt + it0;
     const size_t cs = (size_t)ntx * nt;  // channel stride of src/rec
     T* Fc[3] = {Vx, Vy, Vz};
+    Corners C[3];
+    T f[3][8];
 #pragma unroll
-    for (int ch = 1; ch <= 3; ++ch)
-        interp_one<T>(Fc[ch - 1], nx, ny, nz, X.idx + 3 * (is + ntx * ch), X.w + 8 * (is + ntx * ch),
-                      &X.rec[o + cs * ch], X.src[o + cs * ch]);
+    for (int ch = 0; ch < 3; ++ch) {
+        C[ch] = corners_of(X.idx + 3 * (is + ntx * (ch + 1)), nx, ny, nz);
+        if (C[ch].ok) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) f[ch][q] = Fc[ch][C[ch].c[q]];
+        }
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+        if (C[ch].ok)
+            interp_apply<T>(Fc[ch], C[ch], X.w + 8 * (is + ntx * (ch + 1)), f[ch], &X.rec[o + cs * (ch + 1)],
+                            X.src[o + cs * (ch + 1)]);
     const T* d = X.dir + 4 * is;
     X.vn_rec[o] = X.rec[o + cs] * d[1] + X.rec[o + 2 * cs] * d[2] + X.rec[o + 3 * cs] * d[3];
     if (do_vn2v && it0 + 1 < nt) {

@@ -174,6 +174,16 @@ int adf_solver_run(adf_solver* s, int64_t start_step, int64_t end_step);
  * 2 -> P[:,:,index] (nx,ny); index is 1-based. */
 int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* out);
 
+/* Kernel path and tiling (tuning knobs; results do not depend on them):
+ *   ADF_OPT_PATH     0 auto (fused when nx <= 256), 1 two-pass P/V kernels,
+ *                    2 fused z-marching kernel
+ *   ADF_OPT_TILE_Y   rows per lane of the fused kernel: 0 auto, 2, 4, 8
+ *   ADF_OPT_BLOCK_Z  planes per workgroup of the fused kernel: 0 auto, >= 1 */
+#define ADF_OPT_PATH 1
+#define ADF_OPT_TILE_Y 2
+#define ADF_OPT_BLOCK_Z 3
+int adf_solver_set_option(adf_solver* s, int option, int64_t value);
+
 /* Mean device time per step of the last adf_solver_run (ms), measured
  * with HIP events on the solver's stream. */
 int adf_solver_last_step_ms(adf_solver* s, double* ms);

@@ -18,8 +18,17 @@ pytestmark = pytest.mark.gpu
 
 TOL = {"f64": 1e-12, "ref": 1e-6, "fast": 1e-5}
 
+# Both kernel paths (two-pass P/V kernels and the fused z-marching sweep,
+# with several tile shapes) must give the same results.
+PATHS = [("two-pass", 0, 0), ("fused", 2, 3), ("fused", 4, 0), ("fused", 8, 5)]
 
-def _compare(d, txs, cpml, mask, kind, field=None, gf_pair=None):
+
+@pytest.fixture(params=PATHS, ids=lambda p: f"{p[0]}-ty{p[1]}-kb{p[2]}")
+def path(request):
+    return request.param
+
+
+def _compare(d, txs, cpml, mask, kind, field=None, gf_pair=None, path=("auto", 0, 0)):
     """kind: 'f64' (GPU F64 vs oracle f64), 'ref' (GPU F32_REF vs oracle r32)
     or 'fast' (GPU F32 vs oracle f64)."""
     mode = "r32" if kind == "ref" else "f64"
@@ -29,7 +38,8 @@ def _compare(d, txs, cpml, mask, kind, field=None, gf_pair=None):
     ref.run()
     dtype = np.float64 if kind == "f64" else np.float32
     prec = "ref" if kind == "ref" else "fast"
-    f, g, cp = gpu_run(d, txs, cpml, mask, dtype, prec, field=field, gf=None if gf_pair is None else gf_pair[1])
+    f, g, cp = gpu_run(d, txs, cpml, mask, dtype, prec, field=field, gf=None if gf_pair is None else gf_pair[1],
+                       path=path[0], tile_y=path[1], block_z=path[2])
     tol = TOL[kind]
     errs = {
         "p_rec": rel_l2(g.rec[0], ref.tx["rec"][0]),
@@ -50,31 +60,31 @@ def _compare(d, txs, cpml, mask, kind, field=None, gf_pair=None):
 
 
 @pytest.mark.parametrize("kind", ["f64", "ref", "fast"])
-def test_point_source_cpml(kind):
+def test_point_source_cpml(kind, path):
     d, txs, cpml, mask = build(n=32, nt=90, npml=6, n_rec=24, rec_radius_cells=8)
-    errs = _compare(d, txs, cpml, mask, kind)
+    errs = _compare(d, txs, cpml, mask, kind, path=path)
     if kind in ("f64", "ref"):
         assert errs["p_rec"] == 0.0 and errs["P"] == 0.0  # same operations, same order
 
 
 @pytest.mark.parametrize("kind", ["f64", "ref", "fast"])
-def test_sphere_mask_velocity_source(kind):
+def test_sphere_mask_velocity_source(kind, path):
     d, txs, cpml, mask = build(n=(36, 33, 30), nt=80, npml=5, shape="sphere", shape_params=[5 * 0.00686],
                                n_rec=30, rec_radius_cells=9, src_cells=(8.3, -1.2, 0.7),
                                src_dir=(0.0, 0.6, 0.8, 0.0))
-    _compare(d, txs, cpml, mask, kind)
+    _compare(d, txs, cpml, mask, kind, path=path)
 
 
 @pytest.mark.parametrize("npml,shape", [(0, "none"), (0, "cube"), (1, "none"), (1, "cube"), (2, "cross")])
-def test_cpml_guards(npml, shape):
+def test_cpml_guards(npml, shape, path):
     # update_p_cpml! skips CPML for npml == 1 while update_p_cpml_mask! does not
     # (updates.jl:46-47 vs :333-337); npml == 0 selects update_p!/update_p_mask!.
     d, txs, cpml, mask = build(n=24, nt=60, npml=npml, shape=shape, shape_params=[8 * 0.00686], n_rec=10,
                                rec_radius_cells=7, src_cells=(2.5, 1.5, -0.5))
-    _compare(d, txs, cpml, mask, "f64")
+    _compare(d, txs, cpml, mask, "f64", path=path)
 
 
-def test_overlapping_injectors_keep_sequential_order():
+def test_overlapping_injectors_keep_sequential_order(path):
     # several transceivers inject into shared cells: the result depends on the
     # order of the transceiver loop (derivatives.jl:30-60); levels keep it.
     d, txs, cpml, mask = build(n=28, nt=60, npml=4, n_rec=0)
@@ -85,27 +95,27 @@ def test_overlapping_injectors_keep_sequential_order():
         extra.append((pt, [1.0, 0.3, -0.2, 0.5], rng.standard_normal(d.nt) * 1e-3))
     d, txs, cpml, mask = build(n=28, nt=60, npml=4, n_rec=0, extra=extra)
     for kind in ("f64", "ref"):
-        errs = _compare(d, txs, cpml, mask, kind)
+        errs = _compare(d, txs, cpml, mask, kind, path=path)
         assert errs["p_rec"] == 0.0
 
 
-def test_velocity_stencils_on_mirror_planes():
+def test_velocity_stencils_on_mirror_planes(path):
     # a velocity transceiver touching Vy j<=2 / Vz k<=2 forces the separate
     # mirror pass (forward.jl:134-137 after the transceivers)
     d, txs, cpml, mask = build(n=20, nt=50, npml=0, n_rec=4, rec_radius_cells=4)
     lo = np.array([-d.xmax, -d.ymax, -d.zmax]) + np.array([5.2, 0.7, 0.4]) * d.dx
     extra = [(lo, [0.0, 0.2, 0.7, 0.7], np.sin(np.arange(d.nt) * 0.3) * 1e-4)]
     d, txs, cpml, mask = build(n=20, nt=50, npml=0, n_rec=4, rec_radius_cells=4, extra=extra)
-    _compare(d, txs, cpml, mask, "f64")
+    _compare(d, txs, cpml, mask, "f64", path=path)
 
 
-def test_plane_wave_initial_field_hologram_like():
+def test_plane_wave_initial_field_hologram_like(path):
     # initialField plane wave (simulation.jl:468-518) with surfaces as receivers
     d, txs, cpml, mask = build(n=(40, 24, 24), nt=70, npml=5, shape="cube", shape_params=[6 * 0.00686],
                                n_rec=16, rec_radius_cells=7, src_dir=(0.0, 0.0, 0.0, 0.0))
     f0 = A.initial_field(d, 2000.0, 0.12).astype(np.float64) * np.float32(1.5)
-    _compare(d, txs, cpml, mask, "f64", field=f0)
-    _compare(d, txs, cpml, mask, "ref", field=f0.astype(np.float32))
+    _compare(d, txs, cpml, mask, "f64", field=f0, path=path)
+    _compare(d, txs, cpml, mask, "ref", field=f0.astype(np.float32), path=path)
 
 
 def test_device_mask_bit_exact():
@@ -124,7 +134,7 @@ def test_device_mask_bit_exact():
 
 
 @pytest.mark.parametrize("writeback", [True, False])
-def test_gf_map_pv_from_pin(writeback):
+def test_gf_map_pv_from_pin(writeback, path):
     # outer surface (receivers) drives inner sources through a random causal GF
     rng = np.random.default_rng(11)
     d, txs, cpml, mask = build(n=26, nt=60, npml=4, n_rec=14, rec_radius_cells=7)
@@ -136,7 +146,7 @@ def test_gf_map_pv_from_pin(writeback):
     vp = rng.standard_normal((n_out, n_in, ng)) * 1e-2
     ref_args = ("pv_from_pin", (pp, vp), iRec, iSrc, writeback)
     gf = A.PVFromPin(pp, vp, iRec, iSrc, writeback)
-    errs = _compare(d, txs, cpml, mask, "f64", gf_pair=(ref_args, gf))
+    errs = _compare(d, txs, cpml, mask, "f64", gf_pair=(ref_args, gf), path=path)
     assert errs["src"] <= 1e-12
 
 
@@ -152,3 +162,13 @@ def test_solver_reset_and_rerun_identical():
     s.set_transceivers(g)
     s.run()
     assert np.array_equal(a, s.get_trace(0))
+
+
+@pytest.mark.parametrize("kind", ["f64", "ref", "fast"])
+@pytest.mark.parametrize("n", [(68, 14, 12), (132, 10, 13), (100, 9, 11), (256, 6, 7)])
+def test_wide_rows_vector_lanes(kind, n, path):
+    # nx > 64 puts 2 or 4 cells on each lane of the fused kernel (one wave per
+    # x-row); CPML on every axis, a sphere mask and transceivers near the walls
+    d, txs, cpml, mask = build(n=n, nt=40, npml=2, shape="sphere", shape_params=[3.1 * 0.00686], n_rec=12,
+                               rec_radius_cells=2.2, src_cells=(1.3, 0.4, -0.2), src_dir=(1.0, 0.2, -0.3, 0.1))
+    _compare(d, txs, cpml, mask, kind, path=path)
