@@ -26,9 +26,9 @@ EXPORTS = (
     "adf_solver_set_mask", "adf_solver_build_mask", "adf_solver_get_mask",
     "adf_solver_set_transceivers", "adf_solver_get_trace", "adf_solver_set_trace",
     "adf_solver_set_gf_map", "adf_solver_run", "adf_solver_get_pressure_slice",
-    "adf_solver_last_step_ms", "adf_solver_set_option",
+    "adf_solver_last_step_ms", "adf_solver_set_option", "adf_solver_sweep_stats",
 )
-ADF_OPT_PATH, ADF_OPT_TILE_Y, ADF_OPT_BLOCK_Z = 1, 2, 3
+ADF_OPT_PATH, ADF_OPT_TILE_Y, ADF_OPT_BLOCK_Z, ADF_OPT_TIME_SWEEP = 1, 2, 3, 4
 
 
 class AdfDomain(C.Structure):
@@ -91,6 +91,7 @@ def load():
             "adf_solver_get_pressure_slice": ([p, i32, i64, p], i32),
             "adf_solver_last_step_ms": ([p, C.POINTER(d)], i32),
             "adf_solver_set_option": ([p, i32, i64], i32),
+            "adf_solver_sweep_stats": ([p, C.POINTER(d), C.POINTER(i64)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

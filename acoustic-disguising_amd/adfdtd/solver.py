@@ -243,6 +243,14 @@ class Solver:
         self.set_option(L.ADF_OPT_TILE_Y, tile_y)
         self.set_option(L.ADF_OPT_BLOCK_Z, block_z)
 
+    def sweep_stats(self):
+        """(mean ms of one field-update launch, launches) of the last run with
+        set_option(ADF_OPT_TIME_SWEEP, 1)."""
+        v = C.c_double()
+        n = C.c_int64()
+        L.check(self.lib.adf_solver_sweep_stats(self.h, C.byref(v), C.byref(n)), "sweep_stats")
+        return v.value, n.value
+
     def last_step_ms(self) -> float:
         v = C.c_double()
         L.check(self.lib.adf_solver_last_step_ms(self.h, C.byref(v)), "last_step_ms")

@@ -275,6 +275,11 @@ extern "C" int adf_solver_set_option(adf_solver* s, int option, int64_t value) {
     return solver_set_option(s, option, value);
 }
 
+extern "C" int adf_solver_sweep_stats(adf_solver* s, double* avg_ms, int64_t* launches) {
+    if (!s || !avg_ms || !launches) return fail(ADF_EINVAL, "adf_solver_sweep_stats: null argument");
+    return solver_sweep_stats(s, avg_ms, launches);
+}
+
 extern "C" int adf_solver_last_step_ms(adf_solver* s, double* ms) {
     if (!s || !ms) return fail(ADF_EINVAL, "adf_solver_last_step_ms: null argument");
     *ms = s->last_step_ms;

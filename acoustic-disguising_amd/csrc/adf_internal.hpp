@@ -95,6 +95,12 @@ struct adf_solver {
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_step_ms = 0.0;
+    // optional per-launch timing of the field-update kernel
+    bool time_sweep = false;
+    std::vector<hipEvent_t> sweep_ev;  // pairs (start, stop)
+    int sweep_used = 0;
+    double sweep_avg_ms = 0.0;
+    int64_t sweep_n = 0;
 };
 
 namespace adf {
@@ -113,6 +119,7 @@ void* cur_field(adf_solver* s);
 void* cur_mv(adf_solver* s, int a);
 void* cur_mp(adf_solver* s, int a);
 int solver_set_option(adf_solver* s, int option, int64_t value);
+int solver_sweep_stats(adf_solver* s, double* avg_ms, int64_t* n);
 bool fused_supported(const adf_solver* s);
 void pick_tiles(const adf_solver* s, int* ty, int* kb);
 int solver_build_mask_device(adf_solver* s, int shape, const double* params, const double* xc,

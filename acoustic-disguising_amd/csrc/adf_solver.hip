@@ -129,6 +129,8 @@ void solver_free(adf_solver* s) {
         dfree(s->psib[b]);
     }
     dfree(s->coef); dfree(s->mask); dfree(s->gen_tiles);
+    for (hipEvent_t e : s->sweep_ev) hipEventDestroy(e);
+    s->sweep_ev.clear();
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
 }
@@ -360,6 +362,8 @@ int solver_set_option(adf_solver* s, int option, int64_t value) {
     } else if (option == ADF_OPT_TILE_Y) {
         if (value != 0 && value != 2 && value != 4 && value != 8) return fail(ADF_EINVAL, "ADF_OPT_TILE_Y must be 0, 2, 4 or 8");
         s->tile_y = (int)value;
+    } else if (option == ADF_OPT_TIME_SWEEP) {
+        s->time_sweep = value != 0;
     } else if (option == ADF_OPT_BLOCK_Z) {
         if (value < 0 || value > (1 << 20)) return fail(ADF_EINVAL, "ADF_OPT_BLOCK_Z out of range");
         s->block_z = (int)value;
@@ -554,6 +558,17 @@ static void launch_fused(adf_solver* s, const Fused<T>& F, int cpl, hipStream_t 
     else launch_fused_cpl<T, EXACT, 1>(s, F, st);
 }
 
+// Start/stop events around one field-update launch (ADF_OPT_TIME_SWEEP).
+static void sweep_mark(adf_solver* s, hipStream_t st) {
+    if (!s->time_sweep) return;
+    if ((int)s->sweep_ev.size() <= s->sweep_used) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        s->sweep_ev.push_back(e);
+    }
+    (void)hipEventRecord(s->sweep_ev[s->sweep_used++], st);
+}
+
 // One time step it (1-based) of forward_onestep! on stream st.
 template <typename T, bool EXACT>
 static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int ty, int kb) {
@@ -561,7 +576,9 @@ static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int
     Txs<T> X = make_txs<T>(s);
     if (fused) {
         Fused<T> F = make_fused<T>(s, ty, kb);
+        sweep_mark(s, st);
         launch_fused<T, EXACT>(s, F, fused_cpl(s), st);
+        sweep_mark(s, st);
         s->cur ^= 1;  // the new buffers become current
         Solve<T> S = make_solve<T>(s);
         if (s->ntx) {
@@ -580,9 +597,13 @@ static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int
         const int kz = s->block_z ? s->block_z : KZ;
         const dim3 gp((S.nx - 1 + 63) / 64, (S.ny + 3) / 4, (S.nz + kz - 1) / kz);
         const dim3 gv((S.nx + 63) / 64, (S.ny + 3) / 4, (S.nz + kz - 1) / kz);
+        sweep_mark(s, st);
         hipLaunchKernelGGL((k_update_p<T, EXACT>), gp, blk, 0, st, S, kz);
+        sweep_mark(s, st);
         if (s->ntx) launch_tx_p<T>(s, S, X, it0, st);
+        sweep_mark(s, st);
         hipLaunchKernelGGL((k_update_v<T, EXACT>), gv, blk, 0, st, S, kz);
+        sweep_mark(s, st);
         launch_after_v<T>(s, S, X, it, st);
     }
 }
@@ -597,8 +618,24 @@ static int run_impl(adf_solver* s, int64_t start, int64_t end) {
         int rc = prepare_tiles(s, ty, kb);
         if (rc) return rc;
     }
+    s->sweep_used = 0;
+    s->sweep_avg_ms = 0.0;
+    s->sweep_n = 0;
     HIPCHK(hipEventRecord(s->ev0, st));
-    for (int64_t it = start; it <= end; ++it) step_impl<T, EXACT>(s, it, st, fused, ty, kb);
+    for (int64_t it = start; it <= end; ++it) {
+        step_impl<T, EXACT>(s, it, st, fused, ty, kb);
+        if (s->time_sweep && s->sweep_used >= 4096) {  // bound the event pool: flush
+            HIPCHK(hipStreamSynchronize(st));
+            for (int e = 0; e + 1 < s->sweep_used; e += 2) {
+                float m = 0.f;
+                if (hipEventElapsedTime(&m, s->sweep_ev[e], s->sweep_ev[e + 1]) == hipSuccess) {
+                    s->sweep_avg_ms += m;
+                    s->sweep_n++;
+                }
+            }
+            s->sweep_used = 0;
+        }
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(s->ev1, st));
     HIPCHK(hipEventSynchronize(s->ev1));
@@ -606,6 +643,22 @@ static int run_impl(adf_solver* s, int64_t start, int64_t end) {
     HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
     const int64_t nsteps = end - start + 1;
     s->last_step_ms = nsteps > 0 ? ms / (double)nsteps : 0.0;
+    if (s->time_sweep) {
+        for (int e = 0; e + 1 < s->sweep_used; e += 2) {
+            float m = 0.f;
+            if (hipEventElapsedTime(&m, s->sweep_ev[e], s->sweep_ev[e + 1]) == hipSuccess) {
+                s->sweep_avg_ms += m;
+                s->sweep_n++;
+            }
+        }
+        s->sweep_used = 0;
+    }
+    return ADF_OK;
+}
+
+int solver_sweep_stats(adf_solver* s, double* avg_ms, int64_t* n) {
+    *avg_ms = s->sweep_n ? s->sweep_avg_ms / (double)s->sweep_n : 0.0;
+    *n = s->sweep_n;
     return ADF_OK;
 }
 

@@ -182,7 +182,17 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
 #define ADF_OPT_PATH 1
 #define ADF_OPT_TILE_Y 2
 #define ADF_OPT_BLOCK_Z 3
+/*   ADF_OPT_TIME_SWEEP  1: record HIP events around every launch of the
+ *                        field-update kernel (fused sweep, or the P and V
+ *                        kernels of the two-pass path) for
+ *                        adf_solver_sweep_stats; 0: off (default). */
+#define ADF_OPT_TIME_SWEEP 4
 int adf_solver_set_option(adf_solver* s, int option, int64_t value);
+
+/* Mean device time (ms) of one field-update launch and the number of
+ * launches timed during the last adf_solver_run with ADF_OPT_TIME_SWEEP
+ * on (measurement support; no counterpart in the reference). */
+int adf_solver_sweep_stats(adf_solver* s, double* avg_ms, int64_t* launches);
 
 /* Mean device time per step of the last adf_solver_run (ms), measured
  * with HIP events on the solver's stream. */
