@@ -12,5 +12,6 @@ from .geometry import build_scatterer, build_update_mask, outside_predicate
 from .illumination import c_for_n, fibonacci_sphere, illumination_points, impulsive_wavelet, ricker_wavelet
 from .extrapolation import PFromPin, PVFromPin, PVFromPV
 from .solver import Solver, dtype_code
+from .batch import Batch
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -26,9 +26,12 @@ EXPORTS = (
     "adf_solver_set_mask", "adf_solver_build_mask", "adf_solver_get_mask",
     "adf_solver_set_transceivers", "adf_solver_get_trace", "adf_solver_set_trace",
     "adf_solver_set_gf_map", "adf_solver_run", "adf_solver_get_pressure_slice",
-    "adf_solver_last_step_ms", "adf_solver_set_option", "adf_solver_sweep_stats",
+    "adf_solver_last_step_ms", "adf_solver_set_option", "adf_solver_sweep_stats", "adf_solver_next_step",
+    "adf_batch_create", "adf_batch_destroy", "adf_batch_add", "adf_batch_remove", "adf_batch_size",
+    "adf_batch_run", "adf_batch_set_option", "adf_batch_last_step_ms",
 )
 ADF_OPT_PATH, ADF_OPT_TILE_Y, ADF_OPT_BLOCK_Z, ADF_OPT_TIME_SWEEP = 1, 2, 3, 4
+ADF_BATCH_PERSISTENT, ADF_BATCH_GRAPH_STEPS = 1, 2
 
 
 class AdfDomain(C.Structure):
@@ -92,6 +95,15 @@ def load():
             "adf_solver_last_step_ms": ([p, C.POINTER(d)], i32),
             "adf_solver_set_option": ([p, i32, i64], i32),
             "adf_solver_sweep_stats": ([p, C.POINTER(d), C.POINTER(i64)], i32),
+            "adf_solver_next_step": ([p, C.POINTER(i64)], i32),
+            "adf_batch_create": ([p, C.POINTER(p)], i32),
+            "adf_batch_destroy": ([p], i32),
+            "adf_batch_add": ([p, p], i32),
+            "adf_batch_remove": ([p, p], i32),
+            "adf_batch_size": ([p, C.POINTER(i64)], i32),
+            "adf_batch_run": ([p, i64], i32),
+            "adf_batch_set_option": ([p, i32, i64], i32),
+            "adf_batch_last_step_ms": ([p, C.POINTER(d)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

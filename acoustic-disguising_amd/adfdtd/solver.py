@@ -255,3 +255,9 @@ class Solver:
         v = C.c_double()
         L.check(self.lib.adf_solver_last_step_ms(self.h, C.byref(v)), "last_step_ms")
         return v.value
+
+    def next_step(self) -> int:
+        """The 1-based step this solver takes next (1 after create/reset)."""
+        v = C.c_int64()
+        L.check(self.lib.adf_solver_next_step(self.h, C.byref(v)), "next_step")
+        return v.value

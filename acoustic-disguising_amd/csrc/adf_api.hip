@@ -104,6 +104,7 @@ extern "C" int adf_solver_destroy(adf_solver* s) {
     if (!s) return ADF_OK;
     hipSetDevice(s->ctx->device);
     hipStreamSynchronize(s->ctx->stream);
+    if (s->batch) batch_remove(s->batch, s);
     solver_free(s);
     delete s;
     return ADF_OK;
@@ -162,6 +163,7 @@ extern "C" int adf_solver_get_cpml_memory(adf_solver* s, void* const p[3], void*
 extern "C" int adf_solver_set_mask(adf_solver* s, const uint8_t* m) {
     if (!s) return fail(ADF_EINVAL, "adf_solver_set_mask: null solver");
     HIPCHK(hipSetDevice(s->ctx->device));
+    touch(s);
     if (!m) {
         s->has_mask = false;
         return ADF_OK;
@@ -186,6 +188,7 @@ extern "C" int adf_solver_set_mask(adf_solver* s, const uint8_t* m) {
 extern "C" int adf_solver_build_mask(adf_solver* s, int shape, const double* params, const double* xc,
                                      const double* yc, const double* zc) {
     if (!s) return fail(ADF_EINVAL, "adf_solver_build_mask: null solver");
+    touch(s);
     if (shape == 0) {
         s->has_mask = false;
         return ADF_OK;
@@ -225,6 +228,7 @@ extern "C" int adf_solver_set_transceivers(adf_solver* s, int64_t ntx, const int
         return fail(ADF_EINVAL, "adf_solver_set_transceivers: null arrays");
     if (ntx > (int64_t(1) << 24)) return fail(ADF_EINVAL, "adf_solver_set_transceivers: too many transceivers");
     HIPCHK(hipSetDevice(s->ctx->device));
+    touch(s);
     return solver_set_txs(s, ntx, idx, w, src, dir, vn_src);
 }
 
@@ -245,7 +249,8 @@ extern "C" int adf_solver_set_trace(adf_solver* s, int which, const void* in) {
     HIPCHK(hipSetDevice(s->ctx->device));
     HIPCHK(hipMemcpyAsync(p, in, (size_t)s->ntx * s->dom.nt * s->es, hipMemcpyHostToDevice, s->ctx->stream));
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
-    return ADF_OK;
+    touch(s);
+    return solver_trace_written(s, which, in);
 }
 
 extern "C" int adf_solver_set_gf_map(adf_solver* s, int kind, const double* const* tensors, int64_t nt_gf,
@@ -253,6 +258,7 @@ extern "C" int adf_solver_set_gf_map(adf_solver* s, int kind, const double* cons
                                      const int64_t* iSrc, int64_t n_isrc, int writeback) {
     if (!s) return fail(ADF_EINVAL, "adf_solver_set_gf_map: null solver");
     HIPCHK(hipSetDevice(s->ctx->device));
+    touch(s);
     return solver_set_gf(s, kind, tensors, nt_gf, dim2, dim3, iRec, n_irec, iSrc, n_isrc, writeback);
 }
 
@@ -272,6 +278,7 @@ extern "C" int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t in
 
 extern "C" int adf_solver_set_option(adf_solver* s, int option, int64_t value) {
     if (!s) return fail(ADF_EINVAL, "adf_solver_set_option: null solver");
+    touch(s);
     return solver_set_option(s, option, value);
 }
 
@@ -283,5 +290,74 @@ extern "C" int adf_solver_sweep_stats(adf_solver* s, double* avg_ms, int64_t* la
 extern "C" int adf_solver_last_step_ms(adf_solver* s, double* ms) {
     if (!s || !ms) return fail(ADF_EINVAL, "adf_solver_last_step_ms: null argument");
     *ms = s->last_step_ms;
+    return ADF_OK;
+}
+
+extern "C" int adf_solver_next_step(adf_solver* s, int64_t* it) {
+    if (!s || !it) return fail(ADF_EINVAL, "adf_solver_next_step: null argument");
+    *it = s->next_it;
+    return ADF_OK;
+}
+
+// -------------------------------------------------------------- batch
+
+extern "C" int adf_batch_create(adf_ctx* ctx, adf_batch** out) {
+    if (!ctx || !out) return fail(ADF_EINVAL, "adf_batch_create: null argument");
+    adf_batch* b = new adf_batch();
+    b->ctx = ctx;
+    *out = b;
+    return ADF_OK;
+}
+
+extern "C" int adf_batch_destroy(adf_batch* b) {
+    if (!b) return ADF_OK;
+    hipSetDevice(b->ctx->device);
+    hipStreamSynchronize(b->ctx->stream);
+    batch_free(b);
+    delete b;
+    return ADF_OK;
+}
+
+extern "C" int adf_batch_add(adf_batch* b, adf_solver* s) {
+    if (!b || !s) return fail(ADF_EINVAL, "adf_batch_add: null argument");
+    return batch_add(b, s);
+}
+
+extern "C" int adf_batch_remove(adf_batch* b, adf_solver* s) {
+    if (!b || !s) return fail(ADF_EINVAL, "adf_batch_remove: null argument");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    return batch_remove(b, s);
+}
+
+extern "C" int adf_batch_size(adf_batch* b, int64_t* n) {
+    if (!b || !n) return fail(ADF_EINVAL, "adf_batch_size: null argument");
+    *n = (int64_t)b->members.size();
+    return ADF_OK;
+}
+
+extern "C" int adf_batch_run(adf_batch* b, int64_t nsteps) {
+    if (!b) return fail(ADF_EINVAL, "adf_batch_run: null batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    return batch_run(b, nsteps);
+}
+
+extern "C" int adf_batch_set_option(adf_batch* b, int option, int64_t value) {
+    if (!b) return fail(ADF_EINVAL, "adf_batch_set_option: null batch");
+    if (option == ADF_BATCH_PERSISTENT) {
+        if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_BATCH_PERSISTENT: 0 or 1");
+        b->persistent = value != 0;
+    } else if (option == ADF_BATCH_GRAPH_STEPS) {
+        if (value < 0 || value > 4096) return fail(ADF_EINVAL, "ADF_BATCH_GRAPH_STEPS: 0..4096");
+        b->graph_steps = (int)value;
+    } else {
+        return fail(ADF_EINVAL, "adf_batch_set_option: unknown option");
+    }
+    b->dirty = true;
+    return ADF_OK;
+}
+
+extern "C" int adf_batch_last_step_ms(adf_batch* b, double* ms) {
+    if (!b || !ms) return fail(ADF_EINVAL, "adf_batch_last_step_ms: null argument");
+    *ms = b->last_step_ms;
     return ADF_OK;
 }

@@ -1,0 +1,192 @@
+// Batched engine: many independent solves advanced together, one step at a
+// time, with every per-solve quantity that changes between steps (step
+// index, ping-pong parity) kept on the device so a step can be captured in a
+// hipGraph and replayed.
+//
+// The field sweep of all member solves runs as one persistent tile-queue
+// kernel per cells-per-lane class: a fixed grid of waves takes (solve, tile)
+// entries from an atomic counter, general (boundary) tiles first.  The
+// transceiver levels, the velocity fixups and the rigid mirrors of all
+// members run as one launch each, then k_advance moves every solve to the
+// next step.
+#pragma once
+#include "adf_fused.hpp"
+
+namespace adf {
+
+// Transceivers and field buffers of one member solve.
+template <typename T>
+struct TxDev {
+    Txs<T> X;
+    T* fb[2];  // the two field buffers (P,Vx,Vy,Vz at c*n3)
+    int nx, ny, nz, n3;
+    int fuse_mirror;
+};
+
+// Persistent tile queue: waves take entries from ctr[0] until none are left;
+// the last wave to finish resets the counters for the next launch.
+template <typename T, bool EXACT, int CPL, int TY>
+__global__ __launch_bounds__(64) void k_fused_batch(const Fused<T>* __restrict__ fs, const int* __restrict__ cur,
+                                                    const uint2* __restrict__ tiles, int ntiles, unsigned* ctr,
+                                                    int persistent) {
+    // the tile index is wave-uniform: keep it (and all per-solve loads
+    // derived from it) in scalar registers
+    int t = persistent ? -1 : (int)blockIdx.x;
+    for (;;) {
+        if (persistent) {
+            unsigned v = 0;
+            if ((threadIdx.x & 63) == 0) v = atomicAdd(&ctr[0], 1u);
+            t = __builtin_amdgcn_readfirstlane((int)v);
+        }
+        if (t >= ntiles) break;
+        const uint2 e = tiles[t];
+        const int s = (int)(e.x & 0x7fffffffu);
+        const Fused<T> F = fs[2 * s + cur[s]];
+        if (e.x >> 31) fused_tile<T, EXACT, CPL, TY, true>(F, (int)(e.y & 0xffffu), (int)(e.y >> 16));
+        else fused_tile<T, EXACT, CPL, TY, false>(F, (int)(e.y & 0xffffu), (int)(e.y >> 16));
+        if (!persistent) break;
+    }
+    if (persistent && (threadIdx.x & 63) == 0) {
+        __threadfence();
+        const unsigned d = atomicAdd(&ctr[1], 1u);
+        if (d == gridDim.x - 1) {
+            atomicExch(&ctr[0], 0u);
+            atomicExch(&ctr[1], 0u);
+        }
+    }
+}
+
+// entry = solve << 20 | injector << 19 | transceiver
+template <typename T>
+__global__ void k_tx_p_batch(const TxDev<T>* __restrict__ td, const int* __restrict__ it0s,
+                             const int* __restrict__ cur, const uint32_t* __restrict__ ent, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t e = ent[t];
+    const int s = (int)(e >> 20);
+    const int is = (int)(e & 0x7ffffu);
+    const TxDev<T> D = td[s];
+    const Txs<T>& X = D.X;
+    T* P = D.fb[cur[s] ^ 1];
+    const Corners C = corners_of(X.idx + 3 * is, D.nx, D.ny, D.nz);
+    if (!C.ok) return;
+    T f[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) f[q] = P[C.c[q]];
+    const size_t o = (size_t)is * X.nt + it0s[s];
+    const T* w8 = X.w + 8 * is;
+    if ((e >> 19) & 1u) {
+        interp_apply<T>(P, C, w8, f, &X.rec[o], X.src[o]);
+    } else {
+        T r = X.rec[o];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r = r + w8[q] * f[q];
+        X.rec[o] = r;
+    }
+}
+
+template <typename T>
+__global__ void k_tx_v_batch(const TxDev<T>* __restrict__ td, const int* __restrict__ it0s,
+                             const int* __restrict__ cur, const uint32_t* __restrict__ ent, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t e = ent[t];
+    const int s = (int)(e >> 20);
+    const int is = (int)(e & 0x7ffffu);
+    const bool inj = (e >> 19) & 1u;
+    const TxDev<T> D = td[s];
+    const Txs<T>& X = D.X;
+    T* Fb = D.fb[cur[s] ^ 1];
+    const int ntx = X.ntx, nt = X.nt, it0 = it0s[s];
+    const size_t o = (size_t)is * nt + it0;
+    const size_t cs = (size_t)ntx * nt;
+    Corners C[3];
+    T f[3][8];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        C[ch] = corners_of(X.idx + 3 * (is + ntx * (ch + 1)), D.nx, D.ny, D.nz);
+        T* Fc = Fb + (size_t)(ch + 1) * D.n3;
+        if (C[ch].ok) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) f[ch][q] = Fc[C[ch].c[q]];
+        }
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        if (!C[ch].ok) continue;
+        T* Fc = Fb + (size_t)(ch + 1) * D.n3;
+        const T* w8 = X.w + 8 * (is + ntx * (ch + 1));
+        if (inj) {
+            interp_apply<T>(Fc, C[ch], w8, f[ch], &X.rec[o + cs * (ch + 1)], X.src[o + cs * (ch + 1)]);
+        } else {
+            T r = X.rec[o + cs * (ch + 1)];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) r = r + w8[q] * f[ch][q];
+            X.rec[o + cs * (ch + 1)] = r;
+        }
+    }
+    const T* d = X.dir + 4 * is;
+    X.vn_rec[o] = X.rec[o + cs] * d[1] + X.rec[o + 2 * cs] * d[2] + X.rec[o + 3 * cs] * d[3];
+    if (inj && it0 + 1 < nt) {  // vn2v! into it+1 (forward.jl:130-132)
+        const T vn = X.vn_src[o + 1];
+#pragma unroll
+        for (int ch = 1; ch <= 3; ++ch) X.src[o + 1 + cs * ch] = X.src[o + 1 + cs * ch] + vn * d[ch];
+    }
+}
+
+// entry.x = solve, entry.y = axis << 30 | cell
+template <typename T, bool EXACT>
+__global__ void k_fixup_batch(const Fused<T>* __restrict__ fs, const int* __restrict__ cur,
+                              const uint2* __restrict__ ent, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint2 e = ent[t];
+    const int s = (int)e.x;
+    const Fused<T> F = fs[2 * s + cur[s]];
+    const int ax = (int)(e.y >> 30);
+    const int c = (int)(e.y & 0x3fffffffu);
+    const int nx = F.nx, ny = F.ny, nz = F.nz;
+    const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    const bool open = F.mask ? ((mask_word(F.mask, 1 + ax, F.nwx, ny, nz, i, j, k) >> (i & 31)) & 1u) : true;
+    const T* P1 = F.P1();
+    if (ax == 0) {
+        F.V1(0)[c] = v_new<T, EXACT, 0>(F, i, j, k, F.V0(0)[c], P1[c], P1[c - 1], open);
+    } else if (ax == 1) {
+        const T v = v_new<T, EXACT, 1>(F, i, j, k, F.V0(1)[c], P1[c], P1[c - nx], open);
+        F.V1(1)[c] = v;
+        if (j == 1 && F.fuse_mirror) F.V1(1)[c - nx] = -v;
+    } else {
+        const T v = v_new<T, EXACT, 2>(F, i, j, k, F.V0(2)[c], P1[c], P1[c - nx * ny], open);
+        F.V1(2)[c] = v;
+        if (k == 1 && F.fuse_mirror) F.V1(2)[c - nx * ny] = -v;
+    }
+}
+
+// forward.jl:134-137 for the members whose transceivers touch the mirrored
+// planes; blockIdx.y = index into `solves`.
+template <typename T>
+__global__ void k_mirror_batch(const TxDev<T>* __restrict__ td, const int* __restrict__ cur,
+                               const int* __restrict__ solves) {
+    const int s = solves[blockIdx.y];
+    const TxDev<T> D = td[s];
+    T* F = D.fb[cur[s] ^ 1];
+    T* Vy = F + 2 * (size_t)D.n3;
+    T* Vz = F + 3 * (size_t)D.n3;
+    const int nx = D.nx, ny = D.ny, nz = D.nz;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < max(nx * nz, nx * ny); t += gridDim.x * blockDim.x) {
+        if (t < nx * nz) {
+            const int i = t % nx, k = t / nx;
+            Vy[i + nx * ny * k] = -Vy[i + nx * (1 + ny * k)];
+        }
+        if (t < nx * ny) Vz[t] = -Vz[t + nx * ny];
+    }
+}
+
+__global__ void k_advance(int* it0s, int* cur, int n) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    it0s[s] += 1;
+    cur[s] ^= 1;
+}
+
+}  // namespace adf

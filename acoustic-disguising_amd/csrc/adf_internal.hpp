@@ -26,6 +26,8 @@ struct adf_ctx {
     int num_cu = 256;
 };
 
+struct adf_batch;
+
 struct adf_solver {
     adf_ctx* ctx = nullptr;
     adf_domain dom{};
@@ -61,9 +63,13 @@ struct adf_solver {
     void* tx_dir = nullptr;
     void* tx_vn_src = nullptr;
     void* tx_vn_rec = nullptr;
-    int32_t* order_p = nullptr;
-    int32_t* order_v = nullptr;
+    uint32_t* order_p = nullptr;  // entries: index | injector << 31
+    uint32_t* order_v = nullptr;
     std::vector<int> lev_p;  // level offsets into order_p (size nlev+1)
+    // host copies for rebuilding the schedule: 1-based int32 idx [4][ntx][3]
+    // and per-transceiver injector flags by source buffer
+    std::vector<int32_t> h_idx;
+    std::vector<uint8_t> src_inj_p, src_inj_v[3], vn_inj, gf_inj;
     std::vector<int> lev_v;
     bool fuse_mirror = true;
     // velocity cells next to pressure-injection stencils (fused path)
@@ -95,6 +101,8 @@ struct adf_solver {
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_step_ms = 0.0;
+    int64_t next_it = 1;          // next step a batch run takes (1-based)
+    adf_batch* batch = nullptr;   // batch this solver belongs to, if any
     // optional per-launch timing of the field-update kernel
     bool time_sweep = false;
     std::vector<hipEvent_t> sweep_ev;  // pairs (start, stop)
@@ -103,7 +111,50 @@ struct adf_solver {
     int64_t sweep_n = 0;
 };
 
+// A set of solvers advanced together (include/adfdtd.h adf_batch_*).
+struct adf_batch {
+    adf_ctx* ctx = nullptr;
+    std::vector<adf_solver*> members;
+    int dtype = 0;
+    bool dirty = true;
+    bool persistent = true;
+    int graph_steps = 16;     // steps per captured graph (0: no graph)
+    // device tables (rebuilt when dirty)
+    void* fs = nullptr;       // Fused<T>[2 * n]: [2s + cur] = (old = cur, new = cur ^ 1)
+    void* td = nullptr;       // TxDev<T>[n]
+    int* it0 = nullptr;       // per member: 0-based step index of the next step
+    int* cur = nullptr;       // per member: buffer holding the current field
+    struct Group {
+        int cpl, ty, ntiles, grid;
+        uint2* tiles;
+        unsigned* ctr;
+    };
+    std::vector<Group> groups;
+    uint32_t* txp = nullptr;
+    std::vector<int> txp_lev;
+    uint32_t* txv = nullptr;
+    std::vector<int> txv_lev;
+    uint2* fix = nullptr;
+    int n_fix = 0;
+    int* mirror = nullptr;
+    int n_mirror = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int gexec_steps = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_step_ms = 0.0;
+};
+
 namespace adf {
+// A change to a member's schedule, mask or tiling invalidates its batch's
+// device tables.
+inline void touch(adf_solver* s) {
+    if (s->batch) s->batch->dirty = true;
+}
+int batch_add(adf_batch* b, adf_solver* s);
+int batch_remove(adf_batch* b, adf_solver* s);
+int batch_run(adf_batch* b, int64_t nsteps);
+void batch_free(adf_batch* b);
 int solver_alloc(adf_solver* s, const adf_cpml* cp);
 void solver_free(adf_solver* s);
 int solver_zero_state(adf_solver* s);
@@ -120,6 +171,7 @@ void* cur_mv(adf_solver* s, int a);
 void* cur_mp(adf_solver* s, int a);
 int solver_set_option(adf_solver* s, int option, int64_t value);
 int solver_sweep_stats(adf_solver* s, double* avg_ms, int64_t* n);
+int solver_trace_written(adf_solver* s, int which, const void* data);
 bool fused_supported(const adf_solver* s);
 void pick_tiles(const adf_solver* s, int* ty, int* kb);
 int solver_build_mask_device(adf_solver* s, int shape, const double* params, const double* xc,

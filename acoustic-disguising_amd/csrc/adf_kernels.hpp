@@ -278,30 +278,43 @@ __device__ __forceinline__ void interp_apply(T* F, const Corners& C, const T* w8
     *rec = r;
 }
 
-// Pressure transceivers of one conflict-free level (forward.jl:92).
+// Pressure transceivers of one conflict-free level (forward.jl:92).  order
+// entries carry the injector flag in bit 31: transceivers whose p source is
+// zero for the whole run only record (adding w*0 leaves every value as is).
 template <typename T>
-__global__ void k_tx_p(T* P, int nx, int ny, int nz, Txs<T> X, const int32_t* order, int n, int it0) {
+__global__ void k_tx_p(T* P, int nx, int ny, int nz, Txs<T> X, const uint32_t* order, int n, int it0) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const int is = order[t];
+    const uint32_t e = order[t];
+    const int is = (int)(e & 0x7fffffffu);
     const Corners C = corners_of(X.idx + 3 * is, nx, ny, nz);
     if (!C.ok) return;
     T f[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) f[q] = P[C.c[q]];
     const size_t o = (size_t)is * X.nt + it0;
-    interp_apply<T>(P, C, X.w + 8 * is, f, &X.rec[o], X.src[o]);
+    if (e >> 31) {
+        interp_apply<T>(P, C, X.w + 8 * is, f, &X.rec[o], X.src[o]);
+    } else {
+        const T* w8 = X.w + 8 * is;
+        T r = X.rec[o];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r = r + w8[q] * f[q];
+        X.rec[o] = r;
+    }
 }
 
 // Velocity transceivers of one level (forward.jl:111-115), then v2vn!
 // (forward.jl:120) and, when no GF map is active, vn2v! into it+1
 // (forward.jl:130-132) for the same transceiver.
 template <typename T>
-__global__ void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, Txs<T> X, const int32_t* order,
+__global__ void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, Txs<T> X, const uint32_t* order,
                        int n, int it0, int do_vn2v) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const int is = order[t];
+    const uint32_t e = order[t];
+    const int is = (int)(e & 0x7fffffffu);
+    const bool inj = (e >> 31) != 0;
     const int ntx = X.ntx, nt = X.nt;
     const size_t o = (size_t)is * nt + it0;
     const size_t cs = (size_t)ntx * nt;  // channel stride of src/rec
@@ -317,13 +330,21 @@ __global__ void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, Txs<T> X, co
         }
     }
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch)
-        if (C[ch].ok)
-            interp_apply<T>(Fc[ch], C[ch], X.w + 8 * (is + ntx * (ch + 1)), f[ch], &X.rec[o + cs * (ch + 1)],
-                            X.src[o + cs * (ch + 1)]);
+    for (int ch = 0; ch < 3; ++ch) {
+        if (!C[ch].ok) continue;
+        const T* w8 = X.w + 8 * (is + ntx * (ch + 1));
+        if (inj) {
+            interp_apply<T>(Fc[ch], C[ch], w8, f[ch], &X.rec[o + cs * (ch + 1)], X.src[o + cs * (ch + 1)]);
+        } else {
+            T r = X.rec[o + cs * (ch + 1)];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) r = r + w8[q] * f[ch][q];
+            X.rec[o + cs * (ch + 1)] = r;
+        }
+    }
     const T* d = X.dir + 4 * is;
     X.vn_rec[o] = X.rec[o + cs] * d[1] + X.rec[o + 2 * cs] * d[2] + X.rec[o + 3 * cs] * d[3];
-    if (do_vn2v && it0 + 1 < nt) {
+    if (do_vn2v && inj && it0 + 1 < nt) {
         const T vn = X.vn_src[o + 1];
 #pragma unroll
         for (int ch = 1; ch <= 3; ++ch) X.src[o + 1 + cs * ch] = X.src[o + 1 + cs * ch] + vn * d[ch];

@@ -4,12 +4,14 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <unordered_map>
 #include <vector>
 
 #include "adf_internal.hpp"
 #include "adf_kernels.hpp"
 #include "adf_fused.hpp"
+#include "adf_batch.hpp"
 
 namespace adf {
 
@@ -145,6 +147,7 @@ int solver_zero_state(adf_solver* s) {
         }
     }
     s->cur = 0;
+    s->next_it = 1;
     if (s->ntx) {
         const size_t tr = (size_t)s->ntx * s->dom.nt * s->es;
         HIPCHK(hipMemsetAsync(s->tx_rec, 0, 4 * tr, st));
@@ -153,36 +156,51 @@ int solver_zero_state(adf_solver* s) {
     return ADF_OK;
 }
 
-// Conflict-free levels: a transceiver whose stencil shares a cell with an
-// earlier one goes one level later, so running the levels in order and the
-// members of a level in parallel reproduces the reference's sequential
-// loop over transceivers (derivatives.jl:30-60, F4 of the survey).
+// Conflict-free levels.  Only transceivers that inject (a nonzero source
+// sample on the channel, or sources written by the GF map) change the field;
+// the others only record.  A transceiver must run after every earlier
+// injector that shares a stencil cell with it, and an injector also after
+// every earlier recorder of a shared cell.  Running the levels in order and
+// the members of a level in parallel then reproduces the reference's
+// sequential loop over transceivers (derivatives.jl:30-60) value for value.
+// order entries: transceiver index | (injector << 31).
 static void make_levels(int64_t ntx, int nx, int ny, int nz, const std::vector<int32_t>& idx,
-                        const int* chans, int nch, std::vector<int32_t>& order, std::vector<int>& off) {
-    std::vector<std::unordered_map<int64_t, int>> last(nch);
+                        const std::vector<uint8_t>& inj, const int* chans, int nch, std::vector<uint32_t>& order,
+                        std::vector<int>& off) {
+    std::vector<std::unordered_map<int64_t, int>> last_any(nch), last_inj(nch);
     std::vector<int> lev((size_t)ntx, 1);
     int maxl = 1;
+    auto cells = [&](int64_t is, int ci, int64_t* out) {
+        const int32_t* id = &idx[(size_t)3 * (is + ntx * chans[ci])];
+        const int I = id[0], J = id[1], K = id[2];
+        if (!(I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz)) return false;
+        for (int q = 0; q < 8; ++q)
+            out[q] = (int64_t)(I - 1 + (q & 1)) + (int64_t)nx * ((J - 1 + ((q >> 1) & 1)) + (int64_t)ny * (K - 1 + (q >> 2)));
+        return true;
+    };
+    int64_t cs[8];
     for (int64_t is = 0; is < ntx; ++is) {
         int L = 1;
+        const bool I = inj[is] != 0;
         for (int ci = 0; ci < nch; ++ci) {
-            const int32_t* id = &idx[(size_t)3 * (is + ntx * chans[ci])];
-            const int I = id[0], J = id[1], K = id[2];
-            if (!(I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz)) continue;
+            if (!cells(is, ci, cs)) continue;
+            auto& m = I ? last_any[ci] : last_inj[ci];
             for (int q = 0; q < 8; ++q) {
-                const int64_t cell = (int64_t)(I - 1 + (q & 1)) + (int64_t)nx * ((J - 1 + ((q >> 1) & 1)) + (int64_t)ny * (K - 1 + (q >> 2)));
-                auto it = last[ci].find(cell);
-                if (it != last[ci].end()) L = std::max(L, it->second + 1);
+                auto it = m.find(cs[q]);
+                if (it != m.end()) L = std::max(L, it->second + 1);
             }
         }
         lev[is] = L;
         maxl = std::max(maxl, L);
         for (int ci = 0; ci < nch; ++ci) {
-            const int32_t* id = &idx[(size_t)3 * (is + ntx * chans[ci])];
-            const int I = id[0], J = id[1], K = id[2];
-            if (!(I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz)) continue;
+            if (!cells(is, ci, cs)) continue;
             for (int q = 0; q < 8; ++q) {
-                const int64_t cell = (int64_t)(I - 1 + (q & 1)) + (int64_t)nx * ((J - 1 + ((q >> 1) & 1)) + (int64_t)ny * (K - 1 + (q >> 2)));
-                last[ci][cell] = L;
+                int& a = last_any[ci][cs[q]];
+                a = std::max(a, L);
+                if (I) {
+                    int& b = last_inj[ci][cs[q]];
+                    b = std::max(b, L);
+                }
             }
         }
     }
@@ -191,7 +209,96 @@ static void make_levels(int64_t ntx, int nx, int ny, int nz, const std::vector<i
     for (int L = 1; L <= maxl; ++L) off[L] += off[L - 1];
     order.assign((size_t)ntx, 0);
     std::vector<int> pos(off.begin(), off.end() - 1);
-    for (int64_t is = 0; is < ntx; ++is) order[pos[lev[is] - 1]++] = (int32_t)is;
+    for (int64_t is = 0; is < ntx; ++is)
+        order[pos[lev[is] - 1]++] = (uint32_t)is | (inj[is] ? 0x80000000u : 0u);
+}
+
+// Injector flags from host source data: p channel from src[0]; velocity
+// channels from src[1..3] or vn_src (added into them by vn2v!).
+static void injectors_from(adf_solver* s, int which, const void* data) {
+    const int64_t ntx = s->ntx, nt = s->dom.nt;
+    auto nz_row = [&](const void* base, int64_t is) {
+        if (s->es == 8) {
+            const double* r = (const double*)base + is * nt;
+            for (int64_t t = 0; t < nt; ++t) if (r[t] != 0.0) return true;
+        } else {
+            const float* r = (const float*)base + is * nt;
+            for (int64_t t = 0; t < nt; ++t) if (r[t] != 0.0f) return true;
+        }
+        return false;
+    };
+    for (int64_t is = 0; is < ntx; ++is) {
+        if (which == 0) s->src_inj_p[is] = nz_row(data, is);
+        else if (which >= 1 && which <= 3) s->src_inj_v[which - 1][is] = nz_row(data, is);
+        else s->vn_inj[is] = nz_row(data, is);
+    }
+}
+
+// Levels, fixup cells and the mirror decision from the current injector
+// flags; uploads the schedule.
+int rebuild_schedule(adf_solver* s) {
+    const int64_t ntx = s->ntx;
+    if (!ntx) return ADF_OK;
+    const int nx = (int)s->dom.nx, ny = (int)s->dom.ny, nz = (int)s->dom.nz;
+    hipStream_t st = s->ctx->stream;
+    std::vector<uint8_t> ip(ntx), iv(ntx);
+    for (int64_t is = 0; is < ntx; ++is) {
+        ip[is] = s->src_inj_p[is] || s->gf_inj[is];
+        iv[is] = s->src_inj_v[0][is] || s->src_inj_v[1][is] || s->src_inj_v[2][is] || s->vn_inj[is] || s->gf_inj[is];
+    }
+    std::vector<uint32_t> order;
+    const int cp[1] = {0};
+    make_levels(ntx, nx, ny, nz, s->h_idx, ip, cp, 1, order, s->lev_p);
+    HIPCHK(hipMemcpyAsync(s->order_p, order.data(), (size_t)ntx * 4, hipMemcpyHostToDevice, st));
+    std::vector<uint32_t> order_v;
+    const int cv[3] = {1, 2, 3};
+    make_levels(ntx, nx, ny, nz, s->h_idx, iv, cv, 3, order_v, s->lev_v);
+    HIPCHK(hipMemcpyAsync(s->order_v, order_v.data(), (size_t)ntx * 4, hipMemcpyHostToDevice, st));
+
+    // Velocity cells whose update reads a pressure cell an injector writes:
+    // the fused pass recomputes them after the injection (k_fixup_v).
+    std::vector<uint32_t> cells;
+    for (int64_t is = 0; is < ntx; ++is) {
+        if (!ip[is]) continue;
+        const int32_t* id = &s->h_idx[(size_t)3 * is];
+        const int I = id[0], J = id[1], K = id[2];
+        if (!(I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz)) continue;
+        for (int q = 0; q < 8; ++q) {
+            const int i = I - 1 + (q & 1), j = J - 1 + ((q >> 1) & 1), k = K - 1 + (q >> 2);
+            const uint32_t c = (uint32_t)(i + nx * (j + ny * k));
+            if (i >= 1) cells.push_back(c);
+            if (i + 1 <= nx - 1) cells.push_back(c + 1);
+            if (j >= 1) cells.push_back((1u << 30) | c);
+            if (j + 1 <= ny - 1) cells.push_back((1u << 30) | (c + nx));
+            if (k >= 1) cells.push_back((2u << 30) | c);
+            if (k + 1 <= nz - 1) cells.push_back((2u << 30) | (c + (uint32_t)(nx * ny)));
+        }
+    }
+    std::sort(cells.begin(), cells.end());
+    cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+    HIPCHK(hipStreamSynchronize(st));
+    dfree(s->fix_cells);
+    s->fix_cells = nullptr;
+    s->n_fix = (int)cells.size();
+    if (s->n_fix) {
+        int rc = dmalloc((void**)&s->fix_cells, cells.size() * 4, "fixup list");
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(s->fix_cells, cells.data(), cells.size() * 4, hipMemcpyHostToDevice));
+    }
+
+    // The mirrors can ride inside the V update unless a velocity stencil
+    // touches Vy planes j<=2 or Vz planes k<=2 (1-based).
+    s->fuse_mirror = true;
+    for (int64_t is = 0; is < ntx; ++is) {
+        for (int ch = 2; ch <= 3; ++ch) {
+            const int32_t* id = &s->h_idx[(size_t)3 * (is + ntx * ch)];
+            const int I = id[0], J = id[1], K = id[2];
+            if (!(I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz)) continue;
+            if (ch == 2 && J <= 2) s->fuse_mirror = false;
+            if (ch == 3 && K <= 2) s->fuse_mirror = false;
+        }
+    }
+    return ADF_OK;
 }
 
 int solver_set_txs(adf_solver* s, int64_t ntx, const int64_t* idx, const void* w, const void* src,
@@ -232,59 +339,26 @@ int solver_set_txs(adf_solver* s, int64_t ntx, const int64_t* idx, const void* w
     HIPCHK(hipMemsetAsync(s->tx_rec, 0, (size_t)4 * ntx * nt * es, st));
     HIPCHK(hipMemsetAsync(s->tx_vn_rec, 0, (size_t)ntx * nt * es, st));
 
-    std::vector<int32_t> order;
-    const int cp[1] = {0};
-    make_levels(ntx, nx, ny, nz, id32, cp, 1, order, s->lev_p);
-    HIPCHK(hipMemcpyAsync(s->order_p, order.data(), (size_t)ntx * 4, hipMemcpyHostToDevice, st));
-    std::vector<int32_t> order_v;
-    const int cv[3] = {1, 2, 3};
-    make_levels(ntx, nx, ny, nz, id32, cv, 3, order_v, s->lev_v);
-    HIPCHK(hipMemcpyAsync(s->order_v, order_v.data(), (size_t)ntx * 4, hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
-
-    // Velocity cells whose update reads a pressure-injection corner: the
-    // fused pass recomputes them after the injection (k_fixup_v).
-    {
-        std::vector<uint32_t> cells;
-        cells.reserve((size_t)ntx * 48);
-        for (int64_t is = 0; is < ntx; ++is) {
-            const int32_t* id = &id32[(size_t)3 * is];
-            const int I = id[0], J = id[1], K = id[2];
-            if (!(I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz)) continue;
-            for (int q = 0; q < 8; ++q) {
-                const int i = I - 1 + (q & 1), j = J - 1 + ((q >> 1) & 1), k = K - 1 + (q >> 2);
-                const uint32_t c = (uint32_t)(i + nx * (j + ny * k));
-                if (i >= 1) cells.push_back(c);
-                if (i + 1 <= nx - 1) cells.push_back(c + 1);
-                if (j >= 1) cells.push_back((1u << 30) | c);
-                if (j + 1 <= ny - 1) cells.push_back((1u << 30) | (c + nx));
-                if (k >= 1) cells.push_back((2u << 30) | c);
-                if (k + 1 <= nz - 1) cells.push_back((2u << 30) | (c + (uint32_t)(nx * ny)));
-            }
-        }
-        std::sort(cells.begin(), cells.end());
-        cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
-        s->n_fix = (int)cells.size();
-        if (s->n_fix) {
-            if ((rc = dmalloc((void**)&s->fix_cells, cells.size() * 4, "fixup list"))) return rc;
-            HIPCHK(hipMemcpyAsync(s->fix_cells, cells.data(), cells.size() * 4, hipMemcpyHostToDevice, st));
-            HIPCHK(hipStreamSynchronize(st));
-        }
-    }
-
-    // The mirrors can ride inside the V update unless a velocity stencil
-    // touches Vy planes j<=2 or Vz planes k<=2 (1-based).
-    s->fuse_mirror = true;
-    for (int64_t is = 0; is < ntx; ++is) {
-        for (int ch = 2; ch <= 3; ++ch) {
-            const int32_t* id = &id32[(size_t)3 * (is + ntx * ch)];
-            const int I = id[0], J = id[1], K = id[2];
-            if (!(I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz)) continue;
-            if (ch == 2 && J <= 2) s->fuse_mirror = false;
-            if (ch == 3 && K <= 2) s->fuse_mirror = false;
-        }
-    }
+    s->h_idx = id32;
+    s->src_inj_p.assign(ntx, 0);
+    for (int c = 0; c < 3; ++c) s->src_inj_v[c].assign(ntx, 0);
+    s->vn_inj.assign(ntx, 0);
+    s->gf_inj.assign(ntx, 0);
+    const size_t chb = (size_t)ntx * nt * es;
+    for (int c = 0; c < 4; ++c) injectors_from(s, c, (const char*)src + c * chb);
+    if (vn_src) injectors_from(s, 4, vn_src);
+    if ((rc = rebuild_schedule(s))) return rc;
     return ADF_OK;
+}
+
+// A source buffer was overwritten from the host (set_trace): refresh the
+// injector flags of that buffer and the schedule.
+int solver_trace_written(adf_solver* s, int which, const void* data) {
+    if (which >= 5 && which <= 8) injectors_from(s, which - 5, data);
+    else if (which == 9) injectors_from(s, 4, data);
+    else return ADF_OK;
+    return rebuild_schedule(s);
 }
 
 void* solver_trace_ptr(adf_solver* s, int which) {
@@ -342,6 +416,10 @@ int solver_set_gf(adf_solver* s, int kind, const double* const* tensors, int64_t
     }
     s->gf_kind = kind;
     s->gf_wb = writeback ? 1 : 0;
+    if (writeback) {
+        for (int64_t i = 0; i < n_isrc; ++i) s->gf_inj[iSrc[i] - 1] = 1;
+        if ((rc = rebuild_schedule(s))) return rc;
+    }
     s->gf_ng = nt_gf;
     s->gf_d2 = dim2;
     s->gf_d3 = dim3;
@@ -663,9 +741,12 @@ int solver_sweep_stats(adf_solver* s, double* avg_ms, int64_t* n) {
 }
 
 int solver_run(adf_solver* s, int64_t start, int64_t end) {
-    if (s->dtype == ADF_F64) return run_impl<double, true>(s, start, end);
-    if (s->dtype == ADF_F32_REF) return run_impl<float, true>(s, start, end);
-    return run_impl<float, false>(s, start, end);
+    int rc;
+    if (s->dtype == ADF_F64) rc = run_impl<double, true>(s, start, end);
+    else if (s->dtype == ADF_F32_REF) rc = run_impl<float, true>(s, start, end);
+    else rc = run_impl<float, false>(s, start, end);
+    if (!rc) s->next_it = end + 1;
+    return rc;
 }
 
 template <typename T>
@@ -735,6 +816,304 @@ int solver_build_mask_device(adf_solver* s, int shape, const double* params, con
     hipFree(solid);
     if (e != hipSuccess) return fail(ADF_EDEVICE, std::string("mask build: ") + hipGetErrorString(e));
     return ADF_OK;
+}
+
+
+// ===================================================================== batch
+
+// Host form of the interior/general tile split of prepare_tiles.
+static void tiles_host(const adf_solver* s, int ty, int kb, std::vector<uint32_t>& gen, int& ylo, int& ny_, int& zlo,
+                       int& nz_) {
+    const int ny = (int)s->dom.ny, nz = (int)s->dom.nz, p = (int)s->p;
+    const int tiles_y = (ny + ty - 1) / ty, tiles_z = (nz + kb - 1) / kb;
+    const int jlo = std::max(p + 1, 2), jhi = std::min(ny - p - 1, ny - 1);
+    const int klo = std::max(p + 1, 2), khi = std::min(nz - p - 1, nz - 1);
+    auto yint = [&](int t) { const int j0 = t * ty; return j0 - 1 >= jlo && j0 + ty - 1 < jhi; };
+    auto zint = [&](int t) { const int k0 = t * kb; return k0 - 1 >= klo && k0 + kb - 1 < khi; };
+    int a = -1, b = -1, c = -1, d = -1;
+    for (int t = 0; t < tiles_y; ++t) if (yint(t)) { if (a < 0) a = t; b = t + 1; }
+    for (int t = 0; t < tiles_z; ++t) if (zint(t)) { if (c < 0) c = t; d = t + 1; }
+    if (a < 0 || c < 0) a = b = c = d = 0;
+    gen.clear();
+    for (int tz = 0; tz < tiles_z; ++tz)
+        for (int t = 0; t < tiles_y; ++t)
+            if (!(t >= a && t < b && tz >= c && tz < d)) gen.push_back(((uint32_t)tz << 16) | (uint32_t)t);
+    ylo = a; ny_ = b - a; zlo = c; nz_ = d - c;
+}
+
+static void batch_drop_tables(adf_batch* b) {
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    b->gexec_steps = 0;
+    dfree(b->fs); dfree(b->td); dfree(b->it0); dfree(b->cur); dfree(b->txp); dfree(b->txv); dfree(b->fix);
+    dfree(b->mirror);
+    b->fs = b->td = nullptr;
+    b->it0 = b->cur = b->mirror = nullptr;
+    b->txp = b->txv = nullptr;
+    b->fix = nullptr;
+    for (auto& g : b->groups) { dfree(g.tiles); dfree(g.ctr); }
+    b->groups.clear();
+    b->txp_lev.clear();
+    b->txv_lev.clear();
+    b->n_fix = b->n_mirror = 0;
+}
+
+void batch_free(adf_batch* b) {
+    for (adf_solver* s : b->members) s->batch = nullptr;
+    batch_drop_tables(b);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+}
+
+int batch_add(adf_batch* b, adf_solver* s) {
+    if (s->batch) return fail(ADF_ESTATE, "adf_batch_add: solver already belongs to a batch");
+    if (s->ctx != b->ctx) return fail(ADF_EINVAL, "adf_batch_add: solver uses another context");
+    if (!b->members.empty() && s->dtype != b->dtype) return fail(ADF_EINVAL, "adf_batch_add: batch members share one dtype");
+    if (s->gf_kind) return fail(ADF_EINVAL, "adf_batch_add: solvers with a GF map run alone (adf_solver_run)");
+    if (!fused_supported(s)) return fail(ADF_EINVAL, "adf_batch_add: batched solves need nx <= 256 with nx % (cells per lane) == 0");
+    if ((int64_t)b->members.size() >= 4096) return fail(ADF_EINVAL, "adf_batch_add: at most 4096 members");
+    if (s->ntx >= (1 << 19)) return fail(ADF_EINVAL, "adf_batch_add: at most 524287 transceivers per member");
+    if (b->members.empty()) b->dtype = s->dtype;
+    b->members.push_back(s);
+    s->batch = b;
+    b->dirty = true;
+    return ADF_OK;
+}
+
+int batch_remove(adf_batch* b, adf_solver* s) {
+    auto it = std::find(b->members.begin(), b->members.end(), s);
+    if (it == b->members.end()) return fail(ADF_EINVAL, "adf_batch_remove: not a member");
+    (void)hipStreamSynchronize(b->ctx->stream);
+    b->members.erase(it);
+    s->batch = nullptr;
+    b->dirty = true;
+    return ADF_OK;
+}
+
+template <typename T>
+static int batch_build(adf_batch* b) {
+    batch_drop_tables(b);
+    const int n = (int)b->members.size();
+    if (!n) { b->dirty = false; return ADF_OK; }
+    std::vector<Fused<T>> fs(2 * n);
+    std::vector<TxDev<T>> td(n);
+    std::vector<int> it0(n), cur(n);
+    std::map<std::pair<int, int>, std::vector<uint2>> gen_tiles, int_tiles;
+    // total work decides the z-block: enough tiles for every CU
+    int64_t cells = 0;
+    for (adf_solver* s : b->members) cells += s->n3;
+    for (int m = 0; m < n; ++m) {
+        adf_solver* s = b->members[m];
+        int ty = s->tile_y ? s->tile_y : 2;
+        int kb = s->block_z;
+        if (!kb) {
+            kb = 32;
+            while (kb > 4) {
+                int64_t tiles = 0;
+                for (adf_solver* o : b->members)
+                    tiles += ((o->dom.ny + ty - 1) / ty) * ((o->dom.nz + kb - 1) / kb);
+                if (tiles >= 8LL * b->ctx->num_cu) break;
+                kb /= 2;
+            }
+        }
+        for (int par = 0; par < 2; ++par) {
+            const int save = s->cur;
+            s->cur = par;
+            fs[2 * m + par] = make_fused<T>(s, ty, kb);
+            s->cur = save;
+        }
+        TxDev<T>& D = td[m];
+        D.X = make_txs<T>(s);
+        D.fb[0] = (T*)s->fieldb[0];
+        D.fb[1] = (T*)s->fieldb[1];
+        D.nx = (int)s->dom.nx; D.ny = (int)s->dom.ny; D.nz = (int)s->dom.nz; D.n3 = (int)s->n3;
+        D.fuse_mirror = s->fuse_mirror ? 1 : 0;
+        it0[m] = (int)(s->next_it - 1);
+        cur[m] = s->cur;
+        std::vector<uint32_t> gen;
+        int ylo, nyi, zlo, nzi;
+        tiles_host(s, ty, kb, gen, ylo, nyi, zlo, nzi);
+        const std::pair<int, int> key(fused_cpl(s), ty);
+        for (uint32_t g : gen) gen_tiles[key].push_back(make_uint2((uint32_t)m | 0x80000000u, g));
+        for (int tz = 0; tz < nzi; ++tz)
+            for (int t = 0; t < nyi; ++t)
+                int_tiles[key].push_back(make_uint2((uint32_t)m, ((uint32_t)(zlo + tz) << 16) | (uint32_t)(ylo + t)));
+    }
+    int rc;
+    if ((rc = dmalloc(&b->fs, fs.size() * sizeof(Fused<T>), "batch solves"))) return rc;
+    if ((rc = dmalloc(&b->td, td.size() * sizeof(TxDev<T>), "batch transceivers"))) return rc;
+    if ((rc = dmalloc((void**)&b->it0, n * 4, "batch steps"))) return rc;
+    if ((rc = dmalloc((void**)&b->cur, n * 4, "batch parity"))) return rc;
+    HIPCHK(hipMemcpy(b->fs, fs.data(), fs.size() * sizeof(Fused<T>), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->td, td.data(), td.size() * sizeof(TxDev<T>), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->it0, it0.data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->cur, cur.data(), n * 4, hipMemcpyHostToDevice));
+    for (auto& kv : gen_tiles) int_tiles[kv.first];  // make sure every key exists
+    for (auto& kv : int_tiles) {
+        std::vector<uint2> all = gen_tiles[kv.first];  // general (slower) tiles first
+        all.insert(all.end(), kv.second.begin(), kv.second.end());
+        adf_batch::Group g{};
+        g.cpl = kv.first.first;
+        g.ty = kv.first.second;
+        g.ntiles = (int)all.size();
+        g.grid = b->persistent ? std::min(g.ntiles, 16 * b->ctx->num_cu) : g.ntiles;
+        if ((rc = dmalloc((void**)&g.tiles, all.size() * sizeof(uint2), "batch tiles"))) return rc;
+        if ((rc = dmalloc((void**)&g.ctr, 2 * sizeof(unsigned), "batch counters"))) return rc;
+        HIPCHK(hipMemcpy(g.tiles, all.data(), all.size() * sizeof(uint2), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(g.ctr, 0, 2 * sizeof(unsigned)));
+        b->groups.push_back(g);
+    }
+    // transceiver levels of all members, merged level by level
+    std::vector<std::vector<uint32_t>> lp, lv;
+    std::vector<uint2> fix;
+    std::vector<int> mir;
+    for (int m = 0; m < n; ++m) {
+        adf_solver* s = b->members[m];
+        if (!s->fuse_mirror) mir.push_back(m);
+        if (!s->ntx) continue;
+        std::vector<uint32_t> op(s->ntx), ov(s->ntx);
+        HIPCHK(hipMemcpy(op.data(), s->order_p, s->ntx * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ov.data(), s->order_v, s->ntx * 4, hipMemcpyDeviceToHost));
+        auto merge = [&](const std::vector<uint32_t>& o, const std::vector<int>& lev, std::vector<std::vector<uint32_t>>& out) {
+            if (out.size() + 1 < lev.size()) out.resize(lev.size() - 1);
+            for (size_t L = 0; L + 1 < lev.size(); ++L)
+                for (int q = lev[L]; q < lev[L + 1]; ++q) {
+                    const uint32_t e = o[q];
+                    out[L].push_back(((uint32_t)m << 20) | ((e >> 31) << 19) | (e & 0x7ffffu));
+                }
+        };
+        merge(op, s->lev_p, lp);
+        merge(ov, s->lev_v, lv);
+        if (s->n_fix) {
+            std::vector<uint32_t> fc(s->n_fix);
+            HIPCHK(hipMemcpy(fc.data(), s->fix_cells, s->n_fix * 4, hipMemcpyDeviceToHost));
+            for (uint32_t c : fc) fix.push_back(make_uint2((uint32_t)m, c));
+        }
+    }
+    auto upload_levels = [&](const std::vector<std::vector<uint32_t>>& lv_, uint32_t** dst, std::vector<int>& off) {
+        std::vector<uint32_t> flat;
+        off.assign(1, 0);
+        for (auto& l : lv_) {
+            flat.insert(flat.end(), l.begin(), l.end());
+            off.push_back((int)flat.size());
+        }
+        if (flat.empty()) { off.clear(); return ADF_OK; }
+        int r = dmalloc((void**)dst, flat.size() * 4, "batch transceiver levels");
+        if (r) return r;
+        HIPCHK(hipMemcpy(*dst, flat.data(), flat.size() * 4, hipMemcpyHostToDevice));
+        return ADF_OK;
+    };
+    if ((rc = upload_levels(lp, &b->txp, b->txp_lev))) return rc;
+    if ((rc = upload_levels(lv, &b->txv, b->txv_lev))) return rc;
+    b->n_fix = (int)fix.size();
+    if (b->n_fix) {
+        if ((rc = dmalloc((void**)&b->fix, fix.size() * sizeof(uint2), "batch fixups"))) return rc;
+        HIPCHK(hipMemcpy(b->fix, fix.data(), fix.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    }
+    b->n_mirror = (int)mir.size();
+    if (b->n_mirror) {
+        if ((rc = dmalloc((void**)&b->mirror, mir.size() * 4, "batch mirrors"))) return rc;
+        HIPCHK(hipMemcpy(b->mirror, mir.data(), mir.size() * 4, hipMemcpyHostToDevice));
+    }
+    b->dirty = false;
+    return ADF_OK;
+}
+
+template <typename T, bool EXACT, int CPL>
+static void batch_launch_group(adf_batch* b, const adf_batch::Group& g, hipStream_t st) {
+    const Fused<T>* fs = (const Fused<T>*)b->fs;
+    const int pers = b->persistent ? 1 : 0;
+    if (g.ty == 4)
+        hipLaunchKernelGGL((k_fused_batch<T, EXACT, CPL, 4>), dim3(g.grid), dim3(64), 0, st, fs, b->cur, g.tiles, g.ntiles, g.ctr, pers);
+    else if (g.ty == 8)
+        hipLaunchKernelGGL((k_fused_batch<T, EXACT, CPL, 8>), dim3(g.grid), dim3(64), 0, st, fs, b->cur, g.tiles, g.ntiles, g.ctr, pers);
+    else
+        hipLaunchKernelGGL((k_fused_batch<T, EXACT, CPL, 2>), dim3(g.grid), dim3(64), 0, st, fs, b->cur, g.tiles, g.ntiles, g.ctr, pers);
+}
+
+// The launches of one batched step; every per-solve input is read on the
+// device, so the sequence is the same for every step (graph-capturable).
+template <typename T, bool EXACT>
+static void batch_step(adf_batch* b, hipStream_t st) {
+    for (const auto& g : b->groups) {
+        if (g.cpl == 4) batch_launch_group<T, EXACT, 4>(b, g, st);
+        else if (g.cpl == 2) batch_launch_group<T, EXACT, 2>(b, g, st);
+        else batch_launch_group<T, EXACT, 1>(b, g, st);
+    }
+    const TxDev<T>* td = (const TxDev<T>*)b->td;
+    for (size_t L = 0; L + 1 < b->txp_lev.size(); ++L) {
+        const int n = b->txp_lev[L + 1] - b->txp_lev[L];
+        if (n > 0)
+            hipLaunchKernelGGL((k_tx_p_batch<T>), dim3((n + 255) / 256), dim3(256), 0, st, td, b->it0, b->cur,
+                               b->txp + b->txp_lev[L], n);
+    }
+    if (b->n_fix)
+        hipLaunchKernelGGL((k_fixup_batch<T, EXACT>), dim3((b->n_fix + 255) / 256), dim3(256), 0, st,
+                           (const Fused<T>*)b->fs, b->cur, b->fix, b->n_fix);
+    for (size_t L = 0; L + 1 < b->txv_lev.size(); ++L) {
+        const int n = b->txv_lev[L + 1] - b->txv_lev[L];
+        if (n > 0)
+            hipLaunchKernelGGL((k_tx_v_batch<T>), dim3((n + 255) / 256), dim3(256), 0, st, td, b->it0, b->cur,
+                               b->txv + b->txv_lev[L], n);
+    }
+    if (b->n_mirror)
+        hipLaunchKernelGGL((k_mirror_batch<T>), dim3(64, b->n_mirror), dim3(256), 0, st, td, b->cur, b->mirror);
+    const int n = (int)b->members.size();
+    hipLaunchKernelGGL(k_advance, dim3((n + 255) / 256), dim3(256), 0, st, b->it0, b->cur, n);
+}
+
+template <typename T, bool EXACT>
+static int batch_run_impl(adf_batch* b, int64_t nsteps) {
+    hipStream_t st = b->ctx->stream;
+    if (b->dirty) {
+        int rc = batch_build<T>(b);
+        if (rc) return rc;
+    } else {
+        // members may have been stepped or reset alone: refresh step and parity
+        const int n = (int)b->members.size();
+        std::vector<int> it0(n), cur(n);
+        for (int m = 0; m < n; ++m) { it0[m] = (int)(b->members[m]->next_it - 1); cur[m] = b->members[m]->cur; }
+        HIPCHK(hipMemcpyAsync(b->it0, it0.data(), n * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(b->cur, cur.data(), n * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    if (!b->ev0) { HIPCHK(hipEventCreate(&b->ev0)); HIPCHK(hipEventCreate(&b->ev1)); }
+    int64_t left = nsteps;
+    const int G = b->graph_steps;
+    if (G > 0 && left >= G && !b->gexec) {
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < G; ++i) batch_step<T, EXACT>(b, st);
+        HIPCHK(hipStreamEndCapture(st, &b->graph));
+        HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        b->gexec_steps = G;
+    }
+    HIPCHK(hipEventRecord(b->ev0, st));
+    while (b->gexec && left >= b->gexec_steps) {
+        HIPCHK(hipGraphLaunch(b->gexec, st));
+        left -= b->gexec_steps;
+    }
+    for (; left > 0; --left) batch_step<T, EXACT>(b, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(b->ev1, st));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_step_ms = nsteps > 0 ? ms / (double)nsteps : 0.0;
+    for (adf_solver* s : b->members) {
+        s->next_it += nsteps;
+        if (nsteps & 1) s->cur ^= 1;
+    }
+    return ADF_OK;
+}
+
+int batch_run(adf_batch* b, int64_t nsteps) {
+    if (nsteps < 0) return fail(ADF_EINVAL, "adf_batch_run: nsteps < 0");
+    if (b->members.empty() || nsteps == 0) return ADF_OK;
+    for (adf_solver* s : b->members)
+        if (s->next_it + nsteps - 1 > s->dom.nt) return fail(ADF_EINVAL, "adf_batch_run: a member would step past its nt");
+    if (b->dtype == ADF_F64) return batch_run_impl<double, true>(b, nsteps);
+    if (b->dtype == ADF_F32_REF) return batch_run_impl<float, true>(b, nsteps);
+    return batch_run_impl<float, false>(b, nsteps);
 }
 
 }  // namespace adf

@@ -62,6 +62,7 @@ typedef struct adf_cpml {
 
 typedef struct adf_ctx adf_ctx;
 typedef struct adf_solver adf_solver;
+typedef struct adf_batch adf_batch;
 
 /* ---------------------------------------------------------------- context */
 
@@ -197,6 +198,45 @@ int adf_solver_sweep_stats(adf_solver* s, double* avg_ms, int64_t* launches);
 /* Mean device time per step of the last adf_solver_run (ms), measured
  * with HIP events on the solver's stream. */
 int adf_solver_last_step_ms(adf_solver* s, double* ms);
+
+/* The step the solver takes next (1-based): 1 after create/reset, end+1
+ * after adf_solver_run(start, end), advanced by adf_batch_run. */
+int adf_solver_next_step(adf_solver* s, int64_t* it);
+
+/* ------------------------------------------------------------------ batch
+ *
+ * Many independent solves advanced together on one context.  An addition:
+ * the reference runs its solves one after another (impulsive_gfs loops
+ * run_fdtd! over 2*n_src sources, src/greens.jl:143-164; build_reverb_data
+ * over its sources, src/reverb.jl:265-312).  Each member keeps its own
+ * domain, CPML, mask, transceivers and step counter; a batched step of a
+ * member is the same forward_onestep! (src/kernels/forward.jl:13-139) that
+ * adf_solver_run takes, value for value, whatever else is in the batch.
+ * One step of the batch runs the field sweep of all members as one
+ * persistent tile-queue launch per row-width class, the transceiver levels
+ * of all members as one launch per level, then the velocity fixups and
+ * mirrors; it is captured as a HIP graph and replayed.
+ * Members: same context and dtype, nx <= 256 and nx divisible by its cells
+ * per lane (1 up to nx 64, 2 up to 128, 4 up to 256), no GF map. */
+int adf_batch_create(adf_ctx* ctx, adf_batch** out);
+/* Members are released (not destroyed). */
+int adf_batch_destroy(adf_batch* b);
+int adf_batch_add(adf_batch* b, adf_solver* s);
+int adf_batch_remove(adf_batch* b, adf_solver* s);
+int adf_batch_size(adf_batch* b, int64_t* n);
+/* Advance every member by nsteps steps from its own next step
+ * (adf_solver_next_step); fails if a member would pass its nt.
+ * Synchronous on return. */
+int adf_batch_run(adf_batch* b, int64_t nsteps);
+/*   ADF_BATCH_PERSISTENT   1 (default): a fixed grid of waves drains an
+ *                          atomic tile queue; 0: one workgroup per tile
+ *   ADF_BATCH_GRAPH_STEPS  steps per captured HIP graph (default 16; 0 runs
+ *                          the launches directly) */
+#define ADF_BATCH_PERSISTENT 1
+#define ADF_BATCH_GRAPH_STEPS 2
+int adf_batch_set_option(adf_batch* b, int option, int64_t value);
+/* Mean device time per batched step of the last adf_batch_run (ms). */
+int adf_batch_last_step_ms(adf_batch* b, double* ms);
 
 #ifdef __cplusplus
 }

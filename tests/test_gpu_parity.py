@@ -172,3 +172,38 @@ def test_wide_rows_vector_lanes(kind, n, path):
     d, txs, cpml, mask = build(n=n, nt=40, npml=2, shape="sphere", shape_params=[3.1 * 0.00686], n_rec=12,
                                rec_radius_cells=2.2, src_cells=(1.3, 0.4, -0.2), src_dir=(1.0, 0.2, -0.3, 0.1))
     _compare(d, txs, cpml, mask, kind, path=path)
+
+
+@pytest.mark.parametrize("kind", ["f64", "ref"])
+def test_inject_inner_sources_after_setup(kind, path):
+    # inject_inner_sources! (fdtd_runners.jl:39-45) writes vn_src = C_f*p_drive
+    # and p_src = C_q*vn_drive into an already packed transceiver grid; the
+    # library sees them through adf_solver_set_trace and must treat those
+    # transceivers as injectors (levels, fixups) from then on.
+    rng = np.random.default_rng(2)
+    d, txs, cpml, mask = build(n=30, nt=50, npml=4, n_rec=20, rec_radius_cells=6, src_dir=(0.0, 0.0, 0.0, 0.0))
+    inner = np.arange(1, 21)  # numpy rows of the 20 surface transceivers
+    cf = A.c_for_n(20, 6 * d.dx, d)
+    p_drive = rng.standard_normal((20, d.nt)) * 1e-3
+    vn_drive = rng.standard_normal((20, d.nt)) * 1e-6
+    T = np.float64 if kind == "f64" else np.float32
+    g = A.transceivers_to_grid(d, txs, T)
+    p_src = g.src[0].copy()
+    vn = g.vn_src.copy()
+    p_src[inner] = (cf["C_q"] * vn_drive).astype(T)
+    vn[inner] = (cf["C_f"] * p_drive).astype(T)
+    # oracle with the sources in place from the start
+    ref = oracle_solver(O, d, txs, cpml, mask, "f64" if kind == "f64" else "r32")
+    ref.tx["src"][0][...] = p_src
+    ref.tx["vn_src"][...] = vn
+    ref.run()
+    cp = A.make_cpml(d, cpml.n, cpml.rcoef, cpml.fc, T)
+    s = A.Solver(d, A.dtype_code(T, "ref" if kind == "ref" else "fast"), cp)
+    s.set_path(*path)
+    s.set_transceivers(g)  # zero sources at packing time
+    s.set_trace(A._lib.ADF_SRC_P, p_src)
+    s.set_trace(A._lib.ADF_VN_SRC, vn)
+    s.run()
+    for which, r in ((0, ref.tx["rec"][0]), (4, ref.tx["vn_rec"]), (6, ref.tx["src"][1]), (8, ref.tx["src"][3])):
+        e = rel_l2(s.get_trace(which), r)
+        assert e <= TOL[kind], (which, e)
