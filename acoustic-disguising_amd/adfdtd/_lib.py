@@ -30,7 +30,7 @@ EXPORTS = (
     "adf_batch_create", "adf_batch_destroy", "adf_batch_add", "adf_batch_remove", "adf_batch_size",
     "adf_batch_run", "adf_batch_set_option", "adf_batch_last_step_ms",
 )
-ADF_OPT_PATH, ADF_OPT_TILE_Y, ADF_OPT_BLOCK_Z, ADF_OPT_TIME_SWEEP = 1, 2, 3, 4
+ADF_OPT_PATH, ADF_OPT_TILE_Y, ADF_OPT_BLOCK_Z, ADF_OPT_TIME_SWEEP, ADF_OPT_WAVES, ADF_OPT_BLOCK_Z_GEN = 1, 2, 3, 4, 5, 6
 ADF_BATCH_PERSISTENT, ADF_BATCH_GRAPH_STEPS = 1, 2
 
 

@@ -38,6 +38,10 @@ extern "C" int adf_ctx_create(int device, adf_ctx** out) {
     adf_ctx* c = new adf_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+        if (e != hipSuccess) hipStreamDestroy(c->stream);
+    }
     if (e != hipSuccess) {
         delete c;
         return fail(ADF_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -52,6 +56,8 @@ extern "C" int adf_ctx_destroy(adf_ctx* c) {
     if (!c) return ADF_OK;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    hipStreamSynchronize(c->side);
+    hipStreamDestroy(c->side);
     hipStreamDestroy(c->stream);
     delete c;
     return ADF_OK;

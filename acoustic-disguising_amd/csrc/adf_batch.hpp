@@ -10,7 +10,7 @@
 // members run as one launch each, then k_advance moves every solve to the
 // next step.
 #pragma once
-#include "adf_fused.hpp"
+#include "adf_sweep.hpp"
 
 namespace adf {
 
@@ -23,35 +23,50 @@ struct TxDev {
     int fuse_mirror;
 };
 
-// Persistent tile queue: waves take entries from ctr[0] until none are left;
-// the last wave to finish resets the counters for the next launch.
-template <typename T, bool EXACT, int CPL, int TY>
-__global__ __launch_bounds__(64) void k_fused_batch(const Fused<T>* __restrict__ fs, const int* __restrict__ cur,
-                                                    const uint2* __restrict__ tiles, int ntiles, unsigned* ctr,
-                                                    int persistent) {
-    // the tile index is wave-uniform: keep it (and all per-solve loads
-    // derived from it) in scalar registers
-    int t = persistent ? -1 : (int)blockIdx.x;
-    for (;;) {
-        if (persistent) {
-            unsigned v = 0;
-            if ((threadIdx.x & 63) == 0) v = atomicAdd(&ctr[0], 1u);
-            t = __builtin_amdgcn_readfirstlane((int)v);
-        }
-        if (t >= ntiles) break;
+// Tile entries as xcd_tile describes, in (solve, z-block, y-tile) order;
+// GEN selects the general or the interior tile routine.
+// Non-persistent: W waves per workgroup, block b takes tiles
+// xcd_tile(b) * W + w.  Persistent: a fixed grid of waves; a wave on XCD x
+// takes tiles from chunk x's queue (ctr[x]) and, when it is empty, from the
+// other chunks' queues in turn; the last wave to finish resets the queue
+// heads for the next launch.
+template <typename T, bool EXACT, int CPL, int TY, bool GEN>
+__global__ __launch_bounds__(256) void k_sweep_batch(const Fused<T>* __restrict__ fs, const int* __restrict__ cur,
+                                                     const uint2* __restrict__ tiles, int ntiles, unsigned* ctr,
+                                                     int persistent) {
+    auto run = [&](int t) {
         const uint2 e = tiles[t];
-        const int s = (int)(e.x & 0x7fffffffu);
+        const int s = (int)((e.x >> 16) & 0x7fffu);
         const Fused<T> F = fs[2 * s + cur[s]];
-        if (e.x >> 31) fused_tile<T, EXACT, CPL, TY, true>(F, (int)(e.y & 0xffffu), (int)(e.y >> 16));
-        else fused_tile<T, EXACT, CPL, TY, false>(F, (int)(e.y & 0xffffu), (int)(e.y >> 16));
-        if (!persistent) break;
+        const int tyi = (int)(e.x & 0xffffu), k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0xffffu);
+        const bool masked = (e.x >> 31) != 0;
+        if constexpr (GEN) fused_tile<T, EXACT, CPL, TY, true>(F, tyi, k0, k1, masked);
+        else sweep_int_tile<T, EXACT, CPL, TY>(F, tyi, k0, k1, masked);
+    };
+    if (!persistent) {
+        const int W = (int)(blockDim.x >> 6);
+        const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x) * W + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+        if (t < ntiles) run(t);
+        return;
     }
-    if (persistent && (threadIdx.x & 63) == 0) {
+    const int home = (int)(blockIdx.x & 7);
+    for (int q = 0; q < 8; ++q) {
+        const int x = (home + q) & 7;
+        const int n = xcd_chunk_size(x, ntiles), o = xcd_chunk_start(x, ntiles);
+        for (;;) {
+            unsigned v = 0;
+            if ((threadIdx.x & 63) == 0) v = atomicAdd(&ctr[x], 1u);
+            // wave-uniform: keep the tile index and the per-solve loads scalar
+            const int t = __builtin_amdgcn_readfirstlane((int)v);
+            if (t >= n) break;
+            run(o + t);
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
         __threadfence();
-        const unsigned d = atomicAdd(&ctr[1], 1u);
+        const unsigned d = atomicAdd(&ctr[8], 1u);
         if (d == gridDim.x - 1) {
-            atomicExch(&ctr[0], 0u);
-            atomicExch(&ctr[1], 0u);
+            for (int x = 0; x < 9; ++x) atomicExch(&ctr[x], 0u);
         }
     }
 }

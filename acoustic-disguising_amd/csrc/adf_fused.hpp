@@ -42,8 +42,9 @@ struct Fused {
     const T* coef;
     const uint32_t* mask;
     int nx, ny, nz, p, nwx, n3;
+    int psi_n;  // elements in psi0 / psi1
     int cpml_p, fuse_mirror;
-    int tiles_y, tiles_z, ty, kb;  // tiling of this solve
+    int ty;                        // rows per wave
     int off_mv[3], off_mp[3];      // element offsets of the CPML arrays in psi0/psi1
     double dx, dy, dz, fq, fv;
     float rdx, rdy, rdz, fqf, fvf;
@@ -84,6 +85,69 @@ __device__ __forceinline__ void stv(T* p, const T (&o)[N]) {
         for (int q = 0; q < N; ++q) v[q] = o[q];
         *reinterpret_cast<V*>(p) = v;
     }
+}
+
+// Buffer-resource access for the row loads and stores of the sweep: the
+// descriptor (base, size) sits in scalar registers, the lane's byte offset
+// in one vector register and the row's byte offset in a scalar register, so
+// a row access costs no 64-bit vector address.  Word 3 is the gfx9 default
+// (32-bit data format, no swizzle); accesses past `bytes` read zero.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ Rsrc rsrc_of(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void bld(Rsrc r, int voff, int soff, T (&o)[N]) {
+    constexpr int B = (int)sizeof(T) * N;
+    if constexpr (B == 4) {
+        const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+        __builtin_memcpy(&o[0], &u, 4);
+    } else if constexpr (B == 8) {
+        const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        __builtin_memcpy(&o[0], &u, 8);
+    } else if constexpr (B == 16) {
+        const u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        __builtin_memcpy(&o[0], &u, 16);
+    } else {
+        static_assert(B == 32, "row chunk of 4, 8, 16 or 32 bytes");
+        const u32x4 u0 = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        const u32x4 u1 = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16, soff, 0);
+        __builtin_memcpy(&o[0], &u0, 16);
+        __builtin_memcpy(&o[N / 2], &u1, 16);
+    }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void bst(Rsrc r, int voff, int soff, const T (&o)[N]) {
+    constexpr int B = (int)sizeof(T) * N;
+    if constexpr (B == 4) {
+        uint32_t u;
+        __builtin_memcpy(&u, &o[0], 4);
+        __builtin_amdgcn_raw_buffer_store_b32(u, r, voff, soff, 0);
+    } else if constexpr (B == 8) {
+        u32x2 u;
+        __builtin_memcpy(&u, &o[0], 8);
+        __builtin_amdgcn_raw_buffer_store_b64(u, r, voff, soff, 0);
+    } else if constexpr (B == 16) {
+        u32x4 u;
+        __builtin_memcpy(&u, &o[0], 16);
+        __builtin_amdgcn_raw_buffer_store_b128(u, r, voff, soff, 0);
+    } else {
+        static_assert(B == 32, "row chunk of 4, 8, 16 or 32 bytes");
+        u32x4 u0, u1;
+        __builtin_memcpy(&u0, &o[0], 16);
+        __builtin_memcpy(&u1, &o[N / 2], 16);
+        __builtin_amdgcn_raw_buffer_store_b128(u0, r, voff, soff, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(u1, r, voff + 16, soff, 0);
+    }
+}
+
+__device__ __forceinline__ uint32_t bld_u32(Rsrc r, int voff, int soff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
 }
 
 template <typename T>
@@ -262,7 +326,7 @@ struct FA {
 // indices (always in bounds) instead of guards; values of rows, lanes and
 // planes outside the grid are computed but never stored.
 template <typename T, bool EXACT, int CPL, int TY, bool GEN>
-__device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int tzi) {
+__device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int k0, int k1, bool masked) {
     typedef FA<T, EXACT> M_;
     typedef typename M_::A A;
     const int nx = F.nx, ny = F.ny, nz = F.nz, p = F.p, qv = p + 1;
@@ -272,19 +336,28 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int tzi) 
     const bool lane_ok = i0 < nx;
     const int i0c = lane_ok ? i0 : nx - CPL;  // clamped lane start (loads only)
     const int j0 = tyi * TY;
-    const int k0 = tzi * F.kb;
-    const int k1 = min(k0 + F.kb, nz);
     const int ks = k0 > 0 ? k0 - 1 : k0;  // first plane (halo plane when k0 > 0)
-    const uint32_t* Mk = F.mask;
+    const uint32_t* Mk = masked ? F.mask : nullptr;  // tiles without closed cells skip the mask
     const int sh = i0c & 31;
     const bool cpP = F.cpml_p != 0, cpV = qv > 1;
-    const T* __restrict__ P0 = F.P0();
-    const T* __restrict__ Vx0 = F.V0(0);
-    const T* __restrict__ Vy0 = F.V0(1);
-    const T* __restrict__ Vz0 = F.V0(2);
     const T* __restrict__ psi0 = F.psi0;
     T* __restrict__ psi1 = F.psi1;
     auto crow = [&](int j) { return j < 0 ? 0 : (j >= ny ? ny - 1 : j); };
+    constexpr int ES = (int)sizeof(T);
+    // one descriptor per field buffer (the host keeps 4 n3 ES < 4 GiB for
+    // this kernel); channel c starts at byte c * chb
+    const uint32_t chb = (uint32_t)F.n3 * (uint32_t)ES;
+    const Rsrc rF0 = rsrc_of(F.f0, 4u * chb), rF1 = rsrc_of(F.f1, 4u * chb);
+    const Rsrc rS0 = rsrc_of(psi0, (uint32_t)F.psi_n * ES), rS1 = rsrc_of(psi1, (uint32_t)F.psi_n * ES);
+    const Rsrc rM = rsrc_of(Mk, Mk ? (uint32_t)(4 * F.nwx * ny * nz) * 4u : 0u);
+    const int vl = i0c * ES;         // lane byte offset of loads (clamped lane)
+    const int vs = i0 * ES;          // lane byte offset of stores (lane_ok only)
+    const int vm = (i0c >> 5) * 4;   // lane byte offset of its mask word
+    // byte offset of row j (already clamped) of plane k in a field channel
+    // (unsigned: a Float64 channel may exceed 2 GiB)
+    auto rowb = [&](int j, int k) { return (int)((uint32_t)(nx * j + sxy * k) * (uint32_t)ES); };
+    auto chrow = [&](int c, int rb) { return (int)((uint32_t)c * chb + (uint32_t)rb); };
+    auto maskb = [&](int plane, int j, int k) { return (((plane * nz + k) * ny + j) * F.nwx) * 4; };
 
     // per-lane x-slab data (fixed for the whole sweep)
     int sxP[CPL], lxP[CPL], sxV[CPL], lxV[CPL];
@@ -306,7 +379,7 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int tzi) 
     T p1prev[TY + 1][CPL]; // P1 at the previous plane
 #pragma unroll
     for (int r = 0; r <= TY; ++r) {
-        ldv<T, CPL>(Vz0 + i0c + nx * crow(j0 + r - 1) + sxy * ks, vzc[r]);
+        bld<T, CPL>(rF0, vl, chrow(3, rowb(crow(j0 + r - 1), ks)), vzc[r]);
 #pragma unroll
         for (int q = 0; q < CPL; ++q) p1prev[r][q] = (T)0;
     }
@@ -337,20 +410,20 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int tzi) 
 #pragma unroll
         for (int r = 0; r <= TY + 1; ++r) {
             const int j = j0 + r - 1;
-            const int c = i0c + nx * crow(j) + kc;
-            ldv<T, CPL>(Vy0 + c, vy0[r]);
+            const int jc = crow(j);
+            const int rb = rowb(jc, k);
+            bld<T, CPL>(rF0, vl, chrow(2, rb), vy0[r]);
             if (r > TY) continue;
-            ldv<T, CPL>(P0 + c, p0[r]);
-            ldv<T, CPL>(Vx0 + c, vx0[r]);
-            ldv<T, CPL>(Vz0 + i0c + nx * crow(j) + sxy * kn, vzn[r]);
+            bld<T, CPL>(rF0, vl, rb, p0[r]);
+            bld<T, CPL>(rF0, vl, chrow(1, rb), vx0[r]);
+            bld<T, CPL>(rF0, vl, chrow(3, rowb(jc, kn)), vzn[r]);
             mp_[r] = mvx_[r] = mvy_[r] = mvz_[r] = 0xffffffffu;
             if (Mk) {
-                const int jc = crow(j);
-                mp_[r] = mask_word(Mk, 0, F.nwx, ny, nz, i0c, jc, k) >> sh;
+                mp_[r] = bld_u32(rM, vm, maskb(0, jc, k)) >> sh;
                 if (owned && r > 0) {
-                    mvx_[r] = mask_word(Mk, 1, F.nwx, ny, nz, i0c, jc, k) >> sh;
-                    mvy_[r] = mask_word(Mk, 2, F.nwx, ny, nz, i0c, jc, k) >> sh;
-                    mvz_[r] = mask_word(Mk, 3, F.nwx, ny, nz, i0c, jc, k) >> sh;
+                    mvx_[r] = bld_u32(rM, vm, maskb(1, jc, k)) >> sh;
+                    mvy_[r] = bld_u32(rM, vm, maskb(2, jc, k)) >> sh;
+                    mvz_[r] = bld_u32(rM, vm, maskb(3, jc, k)) >> sh;
                 }
             }
             const bool rowv = j >= 0 && j < ny;
@@ -373,10 +446,10 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int tzi) 
                     if (sxV[q]) mVx[r][q] = psi0[F.mp_off(0) + lxV[q] + qv * (j + ny * (k + nz * (sxV[q] - 1)))];
             }
             if constexpr (GEN) {
-                if (ok && syP[r]) ldv<T, CPL>(psi0 + F.mv_off(1) + i0 + nx * (lyP[r] + p * (k + nz * (syP[r] - 1))), mPy[r]);
-                if (ok && szP && !rj) ldv<T, CPL>(psi0 + F.mv_off(2) + i0 + nx * (j + ny * (lzP + p * (szP - 1))), mPz[r]);
-                if (ok && syV[r]) ldv<T, CPL>(psi0 + F.mp_off(1) + i0 + nx * (lyV[r] + qv * (k + nz * (syV[r] - 1))), mVy[r]);
-                if (ok && owned && r > 0 && szV) ldv<T, CPL>(psi0 + F.mp_off(2) + i0 + nx * (j + ny * (lzV + qv * (szV - 1))), mVz[r]);
+                if (ok && syP[r]) bld<T, CPL>(rS0, vs, (F.mv_off(1) + nx * (lyP[r] + p * (k + nz * (syP[r] - 1)))) * ES, mPy[r]);
+                if (ok && szP && !rj) bld<T, CPL>(rS0, vs, (F.mv_off(2) + nx * (j + ny * (lzP + p * (szP - 1)))) * ES, mPz[r]);
+                if (ok && syV[r]) bld<T, CPL>(rS0, vs, (F.mp_off(1) + nx * (lyV[r] + qv * (k + nz * (syV[r] - 1)))) * ES, mVy[r]);
+                if (ok && owned && r > 0 && szV) bld<T, CPL>(rS0, vs, (F.mp_off(2) + nx * (j + ny * (lzV + qv * (szV - 1)))) * ES, mVz[r]);
             }
         }
 
@@ -425,7 +498,7 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int tzi) 
                 const int j = j0 + r - 1;
                 const T pml = shfl_up1<T>(p1[r][CPL - 1]);  // P1 of the previous lane's last cell
                 if (!lane_ok || j >= ny) continue;
-                const int c = i0 + nx * j + kc;
+                const int rb = rowb(j, k);
                 const bool rj = GEN && (j == ny - 1);
                 if (cpP && !rj && !rk) {
 #pragma unroll
@@ -434,8 +507,8 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int tzi) 
                 }
                 T ayV = (T)0, byV = (T)0;
                 if constexpr (GEN) {
-                    if (syP[r] && !rj) stv<T, CPL>(psi1 + F.mv_off(1) + i0 + nx * (lyP[r] + p * (k + nz * (syP[r] - 1))), mPy[r]);
-                    if (szP && !rj) stv<T, CPL>(psi1 + F.mv_off(2) + i0 + nx * (j + ny * (lzP + p * (szP - 1))), mPz[r]);
+                    if (syP[r] && !rj) bst<T, CPL>(rS1, vs, (F.mv_off(1) + nx * (lyP[r] + p * (k + nz * (syP[r] - 1)))) * ES, mPy[r]);
+                    if (szP && !rj) bst<T, CPL>(rS1, vs, (F.mv_off(2) + nx * (j + ny * (lzP + p * (szP - 1)))) * ES, mPz[r]);
                     if (syV[r]) { ayV = F.cah(lyV[r] + qv * (syV[r] - 1)); byV = F.cbh(lyV[r] + qv * (syV[r] - 1)); }
                 }
                 T vx1[CPL], vy1[CPL], vz1[CPL];
@@ -466,22 +539,22 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int tzi) 
                         vz1[q] = M_::upd(vzc[r][q], F.fv, F.fvf, (mvz_[r] >> q) & 1u, dp);
                     }
                 }
-                stv<T, CPL>(F.P1() + c, p1[r]);
-                stv<T, CPL>(F.V1(0) + c, vx1);
-                if (!GEN || j >= 1 || !F.fuse_mirror) stv<T, CPL>(F.V1(1) + c, vy1);
-                if (!GEN || k >= 1 || !F.fuse_mirror) stv<T, CPL>(F.V1(2) + c, vz1);
+                bst<T, CPL>(rF1, vs, rb, p1[r]);
+                bst<T, CPL>(rF1, vs, chrow(1, rb), vx1);
+                if (!GEN || j >= 1 || !F.fuse_mirror) bst<T, CPL>(rF1, vs, chrow(2, rb), vy1);
+                if (!GEN || k >= 1 || !F.fuse_mirror) bst<T, CPL>(rF1, vs, chrow(3, rb), vz1);
                 if constexpr (GEN) {
                     if (F.fuse_mirror && (j == 1 || k == 1)) {
                         T neg[CPL];
                         if (j == 1) {  // forward.jl:135
 #pragma unroll
                             for (int q = 0; q < CPL; ++q) neg[q] = -vy1[q];
-                            stv<T, CPL>(F.V1(1) + c - nx, neg);
+                            bst<T, CPL>(rF1, vs, chrow(2, (int)((uint32_t)rb - (uint32_t)(nx * ES))), neg);
                         }
                         if (k == 1) {  // forward.jl:136
 #pragma unroll
                             for (int q = 0; q < CPL; ++q) neg[q] = -vz1[q];
-                            stv<T, CPL>(F.V1(2) + c - sxy, neg);
+                            bst<T, CPL>(rF1, vs, chrow(3, (int)((uint32_t)rb - (uint32_t)(sxy * ES))), neg);
                         }
                     }
                 }
@@ -491,8 +564,8 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int tzi) 
                         if (sxV[q]) psi1[F.mp_off(0) + lxV[q] + qv * (j + ny * (k + nz * (sxV[q] - 1)))] = mVx[r][q];
                 }
                 if constexpr (GEN) {
-                    if (syV[r]) stv<T, CPL>(psi1 + F.mp_off(1) + i0 + nx * (lyV[r] + qv * (k + nz * (syV[r] - 1))), mVy[r]);
-                    if (szV) stv<T, CPL>(psi1 + F.mp_off(2) + i0 + nx * (j + ny * (lzV + qv * (szV - 1))), mVz[r]);
+                    if (syV[r]) bst<T, CPL>(rS1, vs, (F.mp_off(1) + nx * (lyV[r] + qv * (k + nz * (syV[r] - 1)))) * ES, mVy[r]);
+                    if (szV) bst<T, CPL>(rS1, vs, (F.mp_off(2) + nx * (j + ny * (lzV + qv * (szV - 1)))) * ES, mVz[r]);
                 }
             }
         }
@@ -506,21 +579,17 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int tzi) 
     }
 }
 
-// One launch for all tiles of a solve: blocks [0, n_gen) take the general
-// tiles of `tiles` (packed tzi << 16 | tyi; started first, they are the
-// slower ones), the rest the interior range [ty_lo, ty_lo+nty) x [tz_lo, ..).
-template <typename T, bool EXACT, int CPL, int TY>
-__global__ __launch_bounds__(64) void k_fused(Fused<T> F, const uint32_t* tiles, int n_gen, int ty_lo, int nty,
-                                              int tz_lo) {
-    const int b = (int)blockIdx.x;
-    if (b < n_gen) {
-        const uint32_t e = tiles[b];
-        fused_tile<T, EXACT, CPL, TY, true>(F, (int)(e & 0xffffu), (int)(e >> 16));
-    } else {
-        const int t = b - n_gen;
-        fused_tile<T, EXACT, CPL, TY, false>(F, ty_lo + t % nty, tz_lo + t / nty);
-    }
-}
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
+// XCD b % 8), each with its own L2.  Block b takes the tiles of chunk b % 8
+// of the list (8 contiguous chunks, chunk x spanning [x q + min(x, r),
+// +q + (x < r)) for n = 8 q + r), so each XCD sweeps a contiguous run of
+// neighbouring tiles, which re-read each other's halo rows from its L2.
+// Tile entries: x = masked << 31 | member << 16 | y-tile, y = first plane
+// << 16 | planes; `masked` is set when a cell of the tile or its halo is
+// closed by the scatterer mask (other tiles skip the mask loads).
+__device__ __forceinline__ int xcd_chunk_start(int x, int n) { return x * (n >> 3) + min(x, n & 7); }
+__device__ __forceinline__ int xcd_chunk_size(int x, int n) { return (n >> 3) + (x < (n & 7) ? 1 : 0); }
+__device__ __forceinline__ int xcd_tile(int b, int n) { return xcd_chunk_start(b & 7, n) + (b >> 3); }
 
 // Velocity cells whose update reads an injected pressure cell (and, for
 // tilings that split rows, seam cells), recomputed from V0 / psi0 and the

@@ -23,6 +23,9 @@ int fail(int code, const std::string& msg);
 struct adf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream of the same (default) priority: the general tiles of a
+    // sweep run on it, concurrently with the interior tiles on `stream`
+    hipStream_t side = nullptr;
     int num_cu = 256;
 };
 
@@ -76,17 +79,20 @@ struct adf_solver {
     uint32_t* fix_cells = nullptr;
     int n_fix = 0;
 
-    // fused-sweep tiling: interior range [ty_lo, ty_lo+nty) x [tz_lo, tz_lo+ntz)
-    // and the list of the remaining (general) tiles, for tiling (tl_ty, tl_kb)
-    int tl_ty = 0, tl_kb = 0;
-    int ty_lo = 0, nty = 0, tz_lo = 0, ntz = 0;
-    uint32_t* gen_tiles = nullptr;
-    int n_gen = 0;
+    // fused-sweep tile lists for tiling (tl_ty, tl_kb): general tiles (CPML
+    // slabs in y/z, boundary planes) and interior tiles
+    int tl_ty = 0, tl_kb = 0, tl_kbg = 0;
+    uint2* tiles_gen = nullptr;
+    uint2* tiles_int = nullptr;
+    int n_gen = 0, n_int = 0;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     // kernel path and tiling (adf_solver_set_option)
     int path = 0;      // 0 auto, 1 two-pass, 2 fused
     int tile_y = 0;    // 0 auto
     int block_z = 0;   // 0 auto
+    int waves = 0;     // waves per workgroup of the fused kernel (0 auto)
+    int block_z_gen = 0;  // planes per general tile (0 auto)
 
     // GF extrapolation map
     int gf_kind = 0;
@@ -124,10 +130,13 @@ struct adf_batch {
     void* td = nullptr;       // TxDev<T>[n]
     int* it0 = nullptr;       // per member: 0-based step index of the next step
     int* cur = nullptr;       // per member: buffer holding the current field
+    // one group per (cells per lane, rows per wave, waves per workgroup);
+    // tile entries x = member, y = tzi << 16 | tyi
     struct Group {
-        int cpl, ty, ntiles, grid;
-        uint2* tiles;
-        unsigned* ctr;
+        int cpl, ty, waves;
+        int n[2], grid[2];      // [0] general tiles, [1] interior tiles
+        uint2* tiles[2];
+        unsigned* ctr[2];       // 8 per-XCD queue heads + a finished-wave count
     };
     std::vector<Group> groups;
     uint32_t* txp = nullptr;
@@ -142,6 +151,7 @@ struct adf_batch {
     hipGraphExec_t gexec = nullptr;
     int gexec_steps = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double last_step_ms = 0.0;
 };
 
@@ -149,6 +159,7 @@ namespace adf {
 // A change to a member's schedule, mask or tiling invalidates its batch's
 // device tables.
 inline void touch(adf_solver* s) {
+    s->tl_ty = 0;  // tile lists carry mask-derived flags: rebuild
     if (s->batch) s->batch->dirty = true;
 }
 int batch_add(adf_batch* b, adf_solver* s);
@@ -173,7 +184,7 @@ int solver_set_option(adf_solver* s, int option, int64_t value);
 int solver_sweep_stats(adf_solver* s, double* avg_ms, int64_t* n);
 int solver_trace_written(adf_solver* s, int which, const void* data);
 bool fused_supported(const adf_solver* s);
-void pick_tiles(const adf_solver* s, int* ty, int* kb);
+void pick_tiles(const adf_solver* s, int* ty, int* kb, int* kbg);
 int solver_build_mask_device(adf_solver* s, int shape, const double* params, const double* xc,
                              const double* yc, const double* zc);
 }  // namespace adf

@@ -5,12 +5,14 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
 #include "adf_internal.hpp"
 #include "adf_kernels.hpp"
 #include "adf_fused.hpp"
+#include "adf_sweep.hpp"
 #include "adf_batch.hpp"
 
 namespace adf {
@@ -130,7 +132,9 @@ void solver_free(adf_solver* s) {
         dfree(s->fieldb[b]);
         dfree(s->psib[b]);
     }
-    dfree(s->coef); dfree(s->mask); dfree(s->gen_tiles);
+    dfree(s->coef); dfree(s->mask); dfree(s->tiles_gen); dfree(s->tiles_int);
+    if (s->ev_fork) hipEventDestroy(s->ev_fork);
+    if (s->ev_join) hipEventDestroy(s->ev_join);
     for (hipEvent_t e : s->sweep_ev) hipEventDestroy(e);
     s->sweep_ev.clear();
     if (s->ev0) hipEventDestroy(s->ev0);
@@ -141,10 +145,9 @@ int solver_zero_state(adf_solver* s) {
     hipStream_t st = s->ctx->stream;
     for (int b = 0; b < 2; ++b) {
         HIPCHK(hipMemsetAsync(s->fieldb[b], 0, 4 * s->n3 * s->es, st));
-        for (int a = 0; a < 3; ++a) {
-            if (s->mv_n[a]) HIPCHK(hipMemsetAsync(s->mvb[b][a], 0, s->mv_n[a] * s->es, st));
-            if (s->mp_n[a]) HIPCHK(hipMemsetAsync(s->mpb[b][a], 0, s->mp_n[a] * s->es, st));
-        }
+        // the whole CPML buffer, padding included (the interior sweep loads
+        // whole rows next to the x slabs)
+        if (s->psi_n) HIPCHK(hipMemsetAsync(s->psib[b], 0, s->psi_n * s->es, st));
     }
     s->cur = 0;
     s->next_it = 1;
@@ -438,13 +441,19 @@ int solver_set_option(adf_solver* s, int option, int64_t value) {
         if (value == 2 && !fused_supported(s)) return fail(ADF_EINVAL, "fused path needs 2 <= nx <= 256");
         s->path = (int)value;
     } else if (option == ADF_OPT_TILE_Y) {
-        if (value != 0 && value != 2 && value != 4 && value != 8) return fail(ADF_EINVAL, "ADF_OPT_TILE_Y must be 0, 2, 4 or 8");
+        if (value != 0 && value != 2 && value != 4) return fail(ADF_EINVAL, "ADF_OPT_TILE_Y must be 0, 2 or 4");
         s->tile_y = (int)value;
     } else if (option == ADF_OPT_TIME_SWEEP) {
         s->time_sweep = value != 0;
     } else if (option == ADF_OPT_BLOCK_Z) {
         if (value < 0 || value > (1 << 20)) return fail(ADF_EINVAL, "ADF_OPT_BLOCK_Z out of range");
         s->block_z = (int)value;
+    } else if (option == ADF_OPT_BLOCK_Z_GEN) {
+        if (value < 0 || value > 65535) return fail(ADF_EINVAL, "ADF_OPT_BLOCK_Z_GEN out of range");
+        s->block_z_gen = (int)value;
+    } else if (option == ADF_OPT_WAVES) {
+        if (value != 0 && value != 1 && value != 2 && value != 4) return fail(ADF_EINVAL, "ADF_OPT_WAVES must be 0, 1, 2 or 4");
+        s->waves = (int)value;
     } else {
         return fail(ADF_EINVAL, "unknown option");
     }
@@ -474,7 +483,7 @@ static Solve<T> make_solve(adf_solver* s) {
 }
 
 template <typename T>
-static Fused<T> make_fused(adf_solver* s, int ty, int kb) {
+static Fused<T> make_fused(adf_solver* s, int ty) {
     Fused<T> F;
     const int nb = s->cur ^ 1;
     F.f0 = (const T*)s->fieldb[s->cur];
@@ -485,15 +494,14 @@ static Fused<T> make_fused(adf_solver* s, int ty, int kb) {
     F.mask = s->has_mask ? s->mask : nullptr;
     F.nx = (int)s->dom.nx; F.ny = (int)s->dom.ny; F.nz = (int)s->dom.nz;
     F.p = (int)s->p; F.nwx = (int)s->nwx; F.n3 = (int)s->n3;
+    F.psi_n = (int)s->psi_n;
     F.cpml_p = s->p > 0 && (s->has_mask || s->p > 1);
     F.fuse_mirror = s->fuse_mirror ? 1 : 0;
     for (int a = 0; a < 3; ++a) {
         F.off_mv[a] = (int)s->psi_off[a];
         F.off_mp[a] = (int)s->psi_off[3 + a];
     }
-    F.ty = ty; F.kb = kb;
-    F.tiles_y = (F.ny + ty - 1) / ty;
-    F.tiles_z = (F.nz + kb - 1) / kb;
+    F.ty = ty;
     F.dx = s->dom.dx; F.dy = s->dom.dy; F.dz = s->dom.dz;
     F.fq = s->dom.fact_m0; F.fv = s->dom.fact_m1;
     F.rdx = (float)(1.0 / s->dom.dx); F.rdy = (float)(1.0 / s->dom.dy); F.rdz = (float)(1.0 / s->dom.dz);
@@ -511,18 +519,22 @@ static Txs<T> make_txs(adf_solver* s) {
 }
 
 // Cells per lane of the fused kernel: one wave64 spans the whole x-row.
+// The fused kernel addresses a field buffer (4 channels) through one
+// buffer descriptor, so it needs 4 n3 sizeof(T) < 4 GiB.
 static int fused_cpl(const adf_solver* s) {
     const int64_t nx = s->dom.nx;
     const int cpl = nx > 128 ? 4 : (nx > 64 ? 2 : 1);
     if (nx > 64 * cpl || nx % cpl != 0) return 0;
+    if (4 * s->n3 * (int64_t)s->es >= (int64_t(1) << 32)) return 0;
     return cpl;
 }
 
 bool fused_supported(const adf_solver* s) { return fused_cpl(s) > 0; }
 
-// Tile shape: TY rows per wave and KB planes per wave, chosen so the launch
-// has several waves per CU while halo work stays small.
-void pick_tiles(const adf_solver* s, int* ty, int* kb) {
+// Tile shape: TY rows per wave, KB planes per interior tile and KBG planes
+// per general tile, chosen so the launch has several waves per CU while
+// halo work stays small; general tiles (slower per plane) get fewer planes.
+void pick_tiles(const adf_solver* s, int* ty, int* kb, int* kbg) {
     int t = s->tile_y ? s->tile_y : 2;
     int k = s->block_z;
     if (!k) {
@@ -534,6 +546,7 @@ void pick_tiles(const adf_solver* s, int* ty, int* kb) {
     }
     *ty = t;
     *kb = k;
+    *kbg = s->block_z_gen ? s->block_z_gen : std::max(2, k / 4);
 }
 
 static constexpr int KZ = 8;
@@ -582,51 +595,148 @@ static void launch_after_v(adf_solver* s, const Solve<T>& S, const Txs<T>& X, in
     }
 }
 
-// Split the (y-tile, z-block) grid into the interior product range, whose
-// rows and planes (with their halos) avoid every y/z CPML slab, the mirrored
-// planes 1,2 and the rigid planes n, and the list of all other tiles.
-static int prepare_tiles(adf_solver* s, int ty, int kb) {
-    if (s->tl_ty == ty && s->tl_kb == kb && (s->gen_tiles || s->n_gen == 0) && s->tl_ty) return ADF_OK;
-    const int ny = (int)s->dom.ny, nz = (int)s->dom.nz, p = (int)s->p;
-    const int tiles_y = (ny + ty - 1) / ty, tiles_z = (nz + kb - 1) / kb;
-    const int jlo = std::max(p + 1, 2), jhi = std::min(ny - p - 1, ny - 1);
-    const int klo = std::max(p + 1, 2), khi = std::min(nz - p - 1, nz - 1);
-    auto yint = [&](int t) { const int j0 = t * ty; return j0 - 1 >= jlo && j0 + ty - 1 < jhi; };
-    auto zint = [&](int t) { const int k0 = t * kb; return k0 - 1 >= klo && k0 + kb - 1 < khi; };
-    int ylo = -1, yhi = -1, zlo = -1, zhi = -1;
-    for (int t = 0; t < tiles_y; ++t) if (yint(t)) { if (ylo < 0) ylo = t; yhi = t + 1; }
-    for (int t = 0; t < tiles_z; ++t) if (zint(t)) { if (zlo < 0) zlo = t; zhi = t + 1; }
-    if (ylo < 0 || zlo < 0) ylo = yhi = zlo = zhi = 0;
-    std::vector<uint32_t> gen;
-    for (int tz = 0; tz < tiles_z; ++tz)
-        for (int t = 0; t < tiles_y; ++t)
-            if (!(t >= ylo && t < yhi && tz >= zlo && tz < zhi)) gen.push_back(((uint32_t)tz << 16) | (uint32_t)t);
-    if (tiles_y > 65535 || tiles_z > 65535) return fail(ADF_EINVAL, "grid too large for the tile list");
-    dfree(s->gen_tiles);
-    s->gen_tiles = nullptr;
-    s->n_gen = (int)gen.size();
-    if (s->n_gen) {
-        int rc = dmalloc((void**)&s->gen_tiles, gen.size() * 4, "tile list");
-        if (rc) return rc;
-        HIPCHK(hipMemcpy(s->gen_tiles, gen.data(), gen.size() * 4, hipMemcpyHostToDevice));
-    }
-    s->ty_lo = ylo; s->nty = yhi - ylo; s->tz_lo = zlo; s->ntz = zhi - zlo;
-    s->tl_ty = ty; s->tl_kb = kb;
+// Split the grid into tiles of ty rows: interior tiles, whose rows and
+// planes (with their halos) avoid every y/z CPML slab, the mirrored planes
+// 1,2 and the rigid planes n, in blocks of kb planes, and general tiles (all
+// others) in blocks of kbg planes; both lists in (z-block, y-tile) order,
+// entries as xcd_tile describes, for member `m`.
+// Rows (j, k) whose cells are all fluid with all faces open (every row
+// when the solve has no mask), from the device mask.
+static int open_rows(const adf_solver* s, std::vector<uint8_t>& open) {
+    const int64_t nx = s->dom.nx, ny = s->dom.ny, nz = s->dom.nz, nwx = s->nwx;
+    open.assign((size_t)(ny * nz), 1);
+    if (!s->has_mask) return ADF_OK;
+    std::vector<uint32_t> bits((size_t)(4 * nwx * ny * nz));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    HIPCHK(hipMemcpy(bits.data(), s->mask, bits.size() * 4, hipMemcpyDeviceToHost));
+    const uint32_t last = (nx % 32) ? ((1u << (nx % 32)) - 1u) : 0xffffffffu;
+    for (int64_t c = 0; c < 4; ++c)
+        for (int64_t r = 0; r < ny * nz; ++r) {
+            const uint32_t* w = &bits[(size_t)((c * ny * nz + r) * nwx)];
+            for (int64_t q = 0; q < nwx; ++q) {
+                const uint32_t want = q == nwx - 1 ? last : 0xffffffffu;
+                if ((w[q] & want) != want) { open[(size_t)r] = 0; break; }
+            }
+        }
     return ADF_OK;
 }
 
+static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, uint32_t m, const std::vector<uint8_t>& open,
+                        std::vector<uint2>& gen, std::vector<uint2>& in) {
+    const int ny = (int)s->dom.ny, nz = (int)s->dom.nz, p = (int)s->p;
+    const int tiles_y = (ny + ty - 1) / ty;
+    const int jlo = std::max(p + 1, 2), jhi = std::min(ny - p - 1, ny - 1);
+    const int klo = std::max(p + 1, 2), khi = std::min(nz - p - 1, nz - 1);
+    auto yint = [&](int t) { const int j0 = t * ty; return j0 - 1 >= jlo && j0 + ty - 1 < jhi; };
+    // interior planes: tiles [k0, k1) with k0 - 1 >= klo and k1 - 1 < khi
+    const int za = klo + 1, zb = khi;
+    // a tile needs the mask if a row of it or of its halo (rows j0-1..j0+ty,
+    // planes k0-1..k1) is not fully open
+    auto tile = [&](int t, int k0, int k1) {
+        bool all_open = true;
+        for (int k = std::max(k0 - 1, 0); k <= std::min(k1, nz - 1) && all_open; ++k)
+            for (int j = std::max(t * ty - 1, 0); j <= std::min(t * ty + ty, ny - 1); ++j)
+                if (!open[(size_t)k * ny + j]) { all_open = false; break; }
+        return make_uint2((all_open ? 0u : 0x80000000u) | (m << 16) | (uint32_t)t,
+                          ((uint32_t)k0 << 16) | (uint32_t)(k1 - k0));
+    };
+    auto blocks = [&](int a, int b, int w, std::vector<std::pair<int, int>>& out) {
+        for (int k = a; k < b; k += w) out.push_back({k, std::min(k + w, b)});
+    };
+    std::vector<std::pair<int, int>> zall, zint, zlow, zhigh;
+    blocks(0, nz, kbg, zall);
+    if (za < zb) {
+        blocks(za, zb, kb, zint);
+        blocks(0, za, kbg, zlow);
+        blocks(zb, nz, kbg, zhigh);
+    }
+    gen.clear();
+    in.clear();
+    // general z-blocks below the interior, then the interior band (general
+    // rows and interior rows), then the blocks above
+    std::vector<std::pair<int, int>> zgen_band;
+    if (za < zb) blocks(za, zb, kbg, zgen_band);
+    if (za >= zb) {
+        for (auto& z : zall)
+            for (int t = 0; t < tiles_y; ++t) gen.push_back(tile(t, z.first, z.second));
+        return;
+    }
+    for (auto& z : zlow)
+        for (int t = 0; t < tiles_y; ++t) gen.push_back(tile(t, z.first, z.second));
+    for (auto& z : zgen_band)
+        for (int t = 0; t < tiles_y; ++t)
+            if (!yint(t)) gen.push_back(tile(t, z.first, z.second));
+    for (auto& z : zhigh)
+        for (int t = 0; t < tiles_y; ++t) gen.push_back(tile(t, z.first, z.second));
+    for (auto& z : zint)
+        for (int t = 0; t < tiles_y; ++t)
+            if (yint(t)) in.push_back(tile(t, z.first, z.second));
+}
+
+static int upload_tiles(const std::vector<uint2>& v, uint2** dst, int* n) {
+    dfree(*dst);
+    *dst = nullptr;
+    *n = (int)v.size();
+    if (v.empty()) return ADF_OK;
+    int rc = dmalloc((void**)dst, v.size() * sizeof(uint2), "tile list");
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(*dst, v.data(), v.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    return ADF_OK;
+}
+
+static int check_tiling(const adf_solver* s, int ty) {
+    if ((s->dom.ny + ty - 1) / ty > 65535 || s->dom.nz > 65535)
+        return fail(ADF_EINVAL, "grid too large for the fused sweep's tile list (ny/tile_y and nz must be < 65536)");
+    return ADF_OK;
+}
+
+static int prepare_tiles(adf_solver* s, int ty, int kb, int kbg) {
+    if (s->tl_ty == ty && s->tl_kb == kb && s->tl_kbg == kbg && (s->tiles_gen || s->tiles_int)) return ADF_OK;
+    int rc = check_tiling(s, ty);
+    if (rc) return rc;
+    std::vector<uint8_t> open;
+    if ((rc = open_rows(s, open))) return rc;
+    std::vector<uint2> gen, in;
+    split_tiles(s, ty, kb, kbg, 0, open, gen, in);
+    if ((rc = upload_tiles(gen, &s->tiles_gen, &s->n_gen))) return rc;
+    if ((rc = upload_tiles(in, &s->tiles_int, &s->n_int))) return rc;
+    if (!s->ev_fork) {
+        HIPCHK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
+    }
+    s->tl_ty = ty;
+    s->tl_kb = kb;
+    s->tl_kbg = kbg;
+    return ADF_OK;
+}
+
+// General tiles on the side stream, interior tiles on the main stream, at
+// the same time (fork/join through events; both streams default priority).
 template <typename T, bool EXACT, int CPL, int TY>
 static void launch_fused_ty(adf_solver* s, const Fused<T>& F, hipStream_t st) {
-    const int n_int = s->nty * s->ntz;
-    hipLaunchKernelGGL((k_fused<T, EXACT, CPL, TY>), dim3((unsigned)(s->n_gen + n_int)), dim3(64), 0, st, F,
-                       (const uint32_t*)s->gen_tiles, s->n_gen, s->ty_lo, s->nty > 0 ? s->nty : 1, s->tz_lo);
+    const int W = s->waves ? s->waves : 1;
+    hipStream_t side = s->ctx->side;
+    const bool fork = s->n_gen && s->n_int;
+    if (fork) {
+        (void)hipEventRecord(s->ev_fork, st);
+        (void)hipStreamWaitEvent(side, s->ev_fork, 0);
+    }
+    if (s->n_gen)
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, TY, true>), dim3((unsigned)((s->n_gen + W - 1) / W)), dim3(64 * W), 0,
+                           fork ? side : st, F, (const uint2*)s->tiles_gen, s->n_gen);
+    if (s->n_int)
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, TY, false>), dim3((unsigned)((s->n_int + W - 1) / W)), dim3(64 * W), 0,
+                           st, F, (const uint2*)s->tiles_int, s->n_int);
+    if (fork) {
+        (void)hipEventRecord(s->ev_join, side);
+        (void)hipStreamWaitEvent(st, s->ev_join, 0);
+    }
 }
 
 template <typename T, bool EXACT, int CPL>
 static void launch_fused_cpl(adf_solver* s, const Fused<T>& F, hipStream_t st) {
-    if (F.ty == 2) launch_fused_ty<T, EXACT, CPL, 2>(s, F, st);
-    else if (F.ty == 8) launch_fused_ty<T, EXACT, CPL, 8>(s, F, st);
-    else launch_fused_ty<T, EXACT, CPL, 4>(s, F, st);
+    if (F.ty == 4) launch_fused_ty<T, EXACT, CPL, 4>(s, F, st);
+    else launch_fused_ty<T, EXACT, CPL, 2>(s, F, st);
 }
 
 template <typename T, bool EXACT>
@@ -649,11 +759,11 @@ static void sweep_mark(adf_solver* s, hipStream_t st) {
 
 // One time step it (1-based) of forward_onestep! on stream st.
 template <typename T, bool EXACT>
-static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int ty, int kb) {
+static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int ty) {
     const int it0 = (int)(it - 1);
     Txs<T> X = make_txs<T>(s);
     if (fused) {
-        Fused<T> F = make_fused<T>(s, ty, kb);
+        Fused<T> F = make_fused<T>(s, ty);
         sweep_mark(s, st);
         launch_fused<T, EXACT>(s, F, fused_cpl(s), st);
         sweep_mark(s, st);
@@ -690,10 +800,10 @@ template <typename T, bool EXACT>
 static int run_impl(adf_solver* s, int64_t start, int64_t end) {
     hipStream_t st = s->ctx->stream;
     const bool fused = s->path == 2 || (s->path == 0 && fused_supported(s));
-    int ty = 4, kb = 16;
+    int ty = 2, kb = 16, kbg = 8;
     if (fused) {
-        pick_tiles(s, &ty, &kb);
-        int rc = prepare_tiles(s, ty, kb);
+        pick_tiles(s, &ty, &kb, &kbg);
+        int rc = prepare_tiles(s, ty, kb, kbg);
         if (rc) return rc;
     }
     s->sweep_used = 0;
@@ -701,7 +811,7 @@ static int run_impl(adf_solver* s, int64_t start, int64_t end) {
     s->sweep_n = 0;
     HIPCHK(hipEventRecord(s->ev0, st));
     for (int64_t it = start; it <= end; ++it) {
-        step_impl<T, EXACT>(s, it, st, fused, ty, kb);
+        step_impl<T, EXACT>(s, it, st, fused, ty);
         if (s->time_sweep && s->sweep_used >= 4096) {  // bound the event pool: flush
             HIPCHK(hipStreamSynchronize(st));
             for (int e = 0; e + 1 < s->sweep_used; e += 2) {
@@ -821,26 +931,6 @@ int solver_build_mask_device(adf_solver* s, int shape, const double* params, con
 
 // ===================================================================== batch
 
-// Host form of the interior/general tile split of prepare_tiles.
-static void tiles_host(const adf_solver* s, int ty, int kb, std::vector<uint32_t>& gen, int& ylo, int& ny_, int& zlo,
-                       int& nz_) {
-    const int ny = (int)s->dom.ny, nz = (int)s->dom.nz, p = (int)s->p;
-    const int tiles_y = (ny + ty - 1) / ty, tiles_z = (nz + kb - 1) / kb;
-    const int jlo = std::max(p + 1, 2), jhi = std::min(ny - p - 1, ny - 1);
-    const int klo = std::max(p + 1, 2), khi = std::min(nz - p - 1, nz - 1);
-    auto yint = [&](int t) { const int j0 = t * ty; return j0 - 1 >= jlo && j0 + ty - 1 < jhi; };
-    auto zint = [&](int t) { const int k0 = t * kb; return k0 - 1 >= klo && k0 + kb - 1 < khi; };
-    int a = -1, b = -1, c = -1, d = -1;
-    for (int t = 0; t < tiles_y; ++t) if (yint(t)) { if (a < 0) a = t; b = t + 1; }
-    for (int t = 0; t < tiles_z; ++t) if (zint(t)) { if (c < 0) c = t; d = t + 1; }
-    if (a < 0 || c < 0) a = b = c = d = 0;
-    gen.clear();
-    for (int tz = 0; tz < tiles_z; ++tz)
-        for (int t = 0; t < tiles_y; ++t)
-            if (!(t >= a && t < b && tz >= c && tz < d)) gen.push_back(((uint32_t)tz << 16) | (uint32_t)t);
-    ylo = a; ny_ = b - a; zlo = c; nz_ = d - c;
-}
-
 static void batch_drop_tables(adf_batch* b) {
     if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
@@ -851,7 +941,8 @@ static void batch_drop_tables(adf_batch* b) {
     b->it0 = b->cur = b->mirror = nullptr;
     b->txp = b->txv = nullptr;
     b->fix = nullptr;
-    for (auto& g : b->groups) { dfree(g.tiles); dfree(g.ctr); }
+    for (auto& g : b->groups)
+        for (int h = 0; h < 2; ++h) { dfree(g.tiles[h]); dfree(g.ctr[h]); }
     b->groups.clear();
     b->txp_lev.clear();
     b->txv_lev.clear();
@@ -863,6 +954,8 @@ void batch_free(adf_batch* b) {
     batch_drop_tables(b);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
 }
 
 int batch_add(adf_batch* b, adf_solver* s) {
@@ -898,17 +991,19 @@ static int batch_build(adf_batch* b) {
     std::vector<Fused<T>> fs(2 * n);
     std::vector<TxDev<T>> td(n);
     std::vector<int> it0(n), cur(n);
-    std::map<std::pair<int, int>, std::vector<uint2>> gen_tiles, int_tiles;
+    std::map<std::tuple<int, int, int>, std::vector<uint2>> gen_tiles, int_tiles;
     // total work decides the z-block: enough tiles for every CU
     int64_t cells = 0;
     for (adf_solver* s : b->members) cells += s->n3;
     for (int m = 0; m < n; ++m) {
         adf_solver* s = b->members[m];
         int ty = s->tile_y ? s->tile_y : 2;
+        // auto: 8-plane interior tiles and 4-plane general tiles, halved
+        // while the whole batch has fewer than 8 tiles per CU
         int kb = s->block_z;
         if (!kb) {
-            kb = 32;
-            while (kb > 4) {
+            kb = 8;
+            while (kb > 2) {
                 int64_t tiles = 0;
                 for (adf_solver* o : b->members)
                     tiles += ((o->dom.ny + ty - 1) / ty) * ((o->dom.nz + kb - 1) / kb);
@@ -916,10 +1011,11 @@ static int batch_build(adf_batch* b) {
                 kb /= 2;
             }
         }
+        const int kbg = s->block_z_gen ? s->block_z_gen : std::max(2, kb / 2);
         for (int par = 0; par < 2; ++par) {
             const int save = s->cur;
             s->cur = par;
-            fs[2 * m + par] = make_fused<T>(s, ty, kb);
+            fs[2 * m + par] = make_fused<T>(s, ty);
             s->cur = save;
         }
         TxDev<T>& D = td[m];
@@ -930,14 +1026,17 @@ static int batch_build(adf_batch* b) {
         D.fuse_mirror = s->fuse_mirror ? 1 : 0;
         it0[m] = (int)(s->next_it - 1);
         cur[m] = s->cur;
-        std::vector<uint32_t> gen;
-        int ylo, nyi, zlo, nzi;
-        tiles_host(s, ty, kb, gen, ylo, nyi, zlo, nzi);
-        const std::pair<int, int> key(fused_cpl(s), ty);
-        for (uint32_t g : gen) gen_tiles[key].push_back(make_uint2((uint32_t)m | 0x80000000u, g));
-        for (int tz = 0; tz < nzi; ++tz)
-            for (int t = 0; t < nyi; ++t)
-                int_tiles[key].push_back(make_uint2((uint32_t)m, ((uint32_t)(zlo + tz) << 16) | (uint32_t)(ylo + t)));
+        int rc0 = check_tiling(s, ty);
+        if (rc0) return rc0;
+        std::vector<uint8_t> open;
+        if ((rc0 = open_rows(s, open))) return rc0;
+        std::vector<uint2> gen, in;
+        split_tiles(s, ty, kb, kbg, (uint32_t)m, open, gen, in);
+        const std::tuple<int, int, int> key(fused_cpl(s), ty, s->waves ? s->waves : 1);
+        auto& G = gen_tiles[key];
+        auto& I = int_tiles[key];
+        G.insert(G.end(), gen.begin(), gen.end());
+        I.insert(I.end(), in.begin(), in.end());
     }
     int rc;
     if ((rc = dmalloc(&b->fs, fs.size() * sizeof(Fused<T>), "batch solves"))) return rc;
@@ -950,17 +1049,23 @@ static int batch_build(adf_batch* b) {
     HIPCHK(hipMemcpy(b->cur, cur.data(), n * 4, hipMemcpyHostToDevice));
     for (auto& kv : gen_tiles) int_tiles[kv.first];  // make sure every key exists
     for (auto& kv : int_tiles) {
-        std::vector<uint2> all = gen_tiles[kv.first];  // general (slower) tiles first
-        all.insert(all.end(), kv.second.begin(), kv.second.end());
         adf_batch::Group g{};
-        g.cpl = kv.first.first;
-        g.ty = kv.first.second;
-        g.ntiles = (int)all.size();
-        g.grid = b->persistent ? std::min(g.ntiles, 16 * b->ctx->num_cu) : g.ntiles;
-        if ((rc = dmalloc((void**)&g.tiles, all.size() * sizeof(uint2), "batch tiles"))) return rc;
-        if ((rc = dmalloc((void**)&g.ctr, 2 * sizeof(unsigned), "batch counters"))) return rc;
-        HIPCHK(hipMemcpy(g.tiles, all.data(), all.size() * sizeof(uint2), hipMemcpyHostToDevice));
-        HIPCHK(hipMemset(g.ctr, 0, 2 * sizeof(unsigned)));
+        g.cpl = std::get<0>(kv.first);
+        g.ty = std::get<1>(kv.first);
+        g.waves = b->persistent ? 1 : std::get<2>(kv.first);
+        const std::vector<uint2>* lists[2] = {&gen_tiles[kv.first], &kv.second};
+        for (int h = 0; h < 2; ++h) {
+            const std::vector<uint2>& L = *lists[h];
+            g.n[h] = (int)L.size();
+            g.grid[h] = b->persistent ? std::min(g.n[h], 16 * b->ctx->num_cu) : (g.n[h] + g.waves - 1) / g.waves;
+            g.tiles[h] = nullptr;
+            g.ctr[h] = nullptr;
+            if (L.empty()) continue;
+            if ((rc = dmalloc((void**)&g.tiles[h], L.size() * sizeof(uint2), "batch tiles"))) return rc;
+            if ((rc = dmalloc((void**)&g.ctr[h], 9 * sizeof(unsigned), "batch counters"))) return rc;
+            HIPCHK(hipMemcpy(g.tiles[h], L.data(), L.size() * sizeof(uint2), hipMemcpyHostToDevice));
+            HIPCHK(hipMemset(g.ctr[h], 0, 9 * sizeof(unsigned)));
+        }
         b->groups.push_back(g);
     }
     // transceiver levels of all members, merged level by level
@@ -1019,27 +1124,42 @@ static int batch_build(adf_batch* b) {
     return ADF_OK;
 }
 
-template <typename T, bool EXACT, int CPL>
-static void batch_launch_group(adf_batch* b, const adf_batch::Group& g, hipStream_t st) {
+template <typename T, bool EXACT, int CPL, int TY>
+static void batch_launch_ty(adf_batch* b, const adf_batch::Group& g, hipStream_t main, hipStream_t side) {
     const Fused<T>* fs = (const Fused<T>*)b->fs;
     const int pers = b->persistent ? 1 : 0;
-    if (g.ty == 4)
-        hipLaunchKernelGGL((k_fused_batch<T, EXACT, CPL, 4>), dim3(g.grid), dim3(64), 0, st, fs, b->cur, g.tiles, g.ntiles, g.ctr, pers);
-    else if (g.ty == 8)
-        hipLaunchKernelGGL((k_fused_batch<T, EXACT, CPL, 8>), dim3(g.grid), dim3(64), 0, st, fs, b->cur, g.tiles, g.ntiles, g.ctr, pers);
-    else
-        hipLaunchKernelGGL((k_fused_batch<T, EXACT, CPL, 2>), dim3(g.grid), dim3(64), 0, st, fs, b->cur, g.tiles, g.ntiles, g.ctr, pers);
+    const dim3 blk(64 * g.waves);
+    if (g.n[0])
+        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, TY, true>), dim3(g.grid[0]), blk, 0, side, fs, b->cur,
+                           g.tiles[0], g.n[0], g.ctr[0], pers);
+    if (g.n[1])
+        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, TY, false>), dim3(g.grid[1]), blk, 0, main, fs, b->cur,
+                           g.tiles[1], g.n[1], g.ctr[1], pers);
 }
 
 // The launches of one batched step; every per-solve input is read on the
 // device, so the sequence is the same for every step (graph-capturable).
+// The general tiles of all groups run on the side stream, concurrently with
+// the interior tiles on the main stream.
 template <typename T, bool EXACT>
 static void batch_step(adf_batch* b, hipStream_t st) {
+    hipStream_t side = b->ctx->side;
+    (void)hipEventRecord(b->ev_fork, st);
+    (void)hipStreamWaitEvent(side, b->ev_fork, 0);
     for (const auto& g : b->groups) {
-        if (g.cpl == 4) batch_launch_group<T, EXACT, 4>(b, g, st);
-        else if (g.cpl == 2) batch_launch_group<T, EXACT, 2>(b, g, st);
-        else batch_launch_group<T, EXACT, 1>(b, g, st);
+        if (g.cpl == 4) {
+            if (g.ty == 4) batch_launch_ty<T, EXACT, 4, 4>(b, g, st, side);
+            else batch_launch_ty<T, EXACT, 4, 2>(b, g, st, side);
+        } else if (g.cpl == 2) {
+            if (g.ty == 4) batch_launch_ty<T, EXACT, 2, 4>(b, g, st, side);
+            else batch_launch_ty<T, EXACT, 2, 2>(b, g, st, side);
+        } else {
+            if (g.ty == 4) batch_launch_ty<T, EXACT, 1, 4>(b, g, st, side);
+            else batch_launch_ty<T, EXACT, 1, 2>(b, g, st, side);
+        }
     }
+    (void)hipEventRecord(b->ev_join, side);
+    (void)hipStreamWaitEvent(st, b->ev_join, 0);
     const TxDev<T>* td = (const TxDev<T>*)b->td;
     for (size_t L = 0; L + 1 < b->txp_lev.size(); ++L) {
         const int n = b->txp_lev[L + 1] - b->txp_lev[L];
@@ -1077,7 +1197,12 @@ static int batch_run_impl(adf_batch* b, int64_t nsteps) {
         HIPCHK(hipMemcpyAsync(b->cur, cur.data(), n * 4, hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
     }
-    if (!b->ev0) { HIPCHK(hipEventCreate(&b->ev0)); HIPCHK(hipEventCreate(&b->ev1)); }
+    if (!b->ev0) {
+        HIPCHK(hipEventCreate(&b->ev0));
+        HIPCHK(hipEventCreate(&b->ev1));
+        HIPCHK(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+    }
     int64_t left = nsteps;
     const int G = b->graph_steps;
     if (G > 0 && left >= G && !b->gexec) {

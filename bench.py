@@ -54,6 +54,45 @@ def build_workload(n=256, npml=16, nt=2050):
     return d, txs, npml
 
 
+def batch_specs(n_solves, sizes, nt, seed=0):
+    """Impulsive-GF solves of the batched workload: solves are split evenly
+    over `sizes` (cells per axis); grid size 128 has a 10-cell CPML, a rigid
+    sphere of radius 16 cells, its source on a Fibonacci sphere of radius 40
+    cells (monopole, compact-Gaussian wavelet, f_3db = 2 kHz) and 100
+    receivers (p and vn) at radius 20 cells; other sizes scale the radii by
+    n/128."""
+    import adfdtd as A
+
+    dx = C_AIR / (2.5 * FC) / 10
+    specs = []
+    per = n_solves // len(sizes)
+    for n in sizes:
+        xmax = (n - 1) * dx / 2
+        d0 = A.make_domain(1.0, xmax, xmax, xmax, n, n, n, C_AIR, RHO_AIR, 0.5)
+        d = A.make_domain((nt - 0.5) * d0.dt, xmax, xmax, xmax, n, n, n, C_AIR, RHO_AIR, 0.5)
+        sc = n / 128
+        src, _ = A.fibonacci_sphere(per, 40 * sc * d.dx)
+        rec, rn = A.fibonacci_sphere(100, 20 * sc * d.dx)
+        tf, _ = A.impulsive_wavelet(d, FC)
+        for i in range(per):
+            specs.append({"d": d, "npml": 10, "sphere": 16 * sc * d.dx, "src": src[i], "tf": tf, "rec": rec,
+                          "rec_n": rn})
+    return specs
+
+
+def make_batch_solver(sp, dtype=np.float32):
+    import adfdtd as A
+
+    d = sp["d"]
+    s = A.Solver(d, A.dtype_code(dtype, "fast"), A.make_cpml(d, sp["npml"], 1e-4, FC, dtype))
+    s.build_mask("sphere", [sp["sphere"]])
+    z = np.zeros(d.nt)
+    txs = [A.make_transceiver(d, sp["src"], [1.0, 0.0, 0.0, 0.0], sp["tf"])]
+    txs += [A.make_transceiver(d, sp["rec"][i], np.r_[0.0, sp["rec_n"][i]], z) for i in range(len(sp["rec"]))]
+    s.set_transceivers(A.transceivers_to_grid(d, txs, dtype))
+    return s
+
+
 def algorithmic_bytes_per_step(n, p, es, masked=True):
     """8 n^3 field words read+written, the CPML memory touched (read+write),
     and a 1-bit scatterer bitmap."""

@@ -178,7 +178,7 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
 /* Kernel path and tiling (tuning knobs; results do not depend on them):
  *   ADF_OPT_PATH     0 auto (fused when nx <= 256), 1 two-pass P/V kernels,
  *                    2 fused z-marching kernel
- *   ADF_OPT_TILE_Y   rows per lane of the fused kernel: 0 auto, 2, 4, 8
+ *   ADF_OPT_TILE_Y   rows per wave of the fused kernel: 0 auto, 2, 4
  *   ADF_OPT_BLOCK_Z  planes per workgroup of the fused kernel: 0 auto, >= 1 */
 #define ADF_OPT_PATH 1
 #define ADF_OPT_TILE_Y 2
@@ -188,6 +188,12 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
  *                        kernels of the two-pass path) for
  *                        adf_solver_sweep_stats; 0: off (default). */
 #define ADF_OPT_TIME_SWEEP 4
+/*   ADF_OPT_WAVES    waves per workgroup of the fused kernel: 0 auto, 1, 2, 4
+ *                    (a workgroup takes that many y-neighbouring tiles) */
+#define ADF_OPT_WAVES 5
+/*   ADF_OPT_BLOCK_Z_GEN  planes per tile of the fused kernel's general tiles
+ *                        (CPML slabs in y/z, boundary planes): 0 auto, >= 1 */
+#define ADF_OPT_BLOCK_Z_GEN 6
 int adf_solver_set_option(adf_solver* s, int option, int64_t value);
 
 /* Mean device time (ms) of one field-update launch and the number of

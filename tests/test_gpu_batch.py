@@ -17,13 +17,13 @@ pytestmark = pytest.mark.gpu
 TRACES = range(10)
 
 
-def make_solver(case, dtype, prec="fast", tile_y=0, block_z=0):
+def make_solver(case, dtype, prec="fast", tile_y=0, block_z=0, waves=0):
     d, txs, cpml, mask = case
     s = A.Solver(d, A.dtype_code(dtype, prec), A.make_cpml(d, cpml.n, cpml.rcoef, cpml.fc, dtype))
     s.set_mask(mask)
     s.set_transceivers(A.transceivers_to_grid(d, txs, dtype))
-    if tile_y or block_z:
-        s.set_path("fused", tile_y, block_z)
+    if tile_y or block_z or waves:
+        s.set_path("fused", tile_y, block_z, waves)
     return s
 
 
@@ -134,7 +134,8 @@ def test_tilings_do_not_change_results():
     for s in ref:
         s.run(1, s.dom.nt)
     b = A.Batch()
-    mem = [make_solver(c, np.float32, tile_y=ty, block_z=kb) for c, (ty, kb) in zip(cs, [(4, 3), (8, 0), (2, 7)])]
+    mem = [make_solver(c, np.float32, tile_y=ty, block_z=kb, waves=w)
+           for c, (ty, kb, w) in zip(cs, [(4, 3, 2), (4, 0, 1), (2, 7, 4)])]
     for s in mem:
         b.add(s)
     b.run(cs[0][0].nt)

@@ -19,16 +19,17 @@ pytestmark = pytest.mark.gpu
 TOL = {"f64": 1e-12, "ref": 1e-6, "fast": 1e-5}
 
 # Both kernel paths (two-pass P/V kernels and the fused z-marching sweep,
-# with several tile shapes) must give the same results.
-PATHS = [("two-pass", 0, 0), ("fused", 2, 3), ("fused", 4, 0), ("fused", 8, 5)]
+# with several tile shapes and waves per workgroup) must give the same
+# results.
+PATHS = [("two-pass", 0, 0, 0), ("fused", 2, 3, 1), ("fused", 4, 0, 2), ("fused", 2, 5, 4)]
 
 
-@pytest.fixture(params=PATHS, ids=lambda p: f"{p[0]}-ty{p[1]}-kb{p[2]}")
+@pytest.fixture(params=PATHS, ids=lambda p: f"{p[0]}-ty{p[1]}-kb{p[2]}-w{p[3]}")
 def path(request):
     return request.param
 
 
-def _compare(d, txs, cpml, mask, kind, field=None, gf_pair=None, path=("auto", 0, 0)):
+def _compare(d, txs, cpml, mask, kind, field=None, gf_pair=None, path=("auto", 0, 0, 0)):
     """kind: 'f64' (GPU F64 vs oracle f64), 'ref' (GPU F32_REF vs oracle r32)
     or 'fast' (GPU F32 vs oracle f64)."""
     mode = "r32" if kind == "ref" else "f64"
@@ -39,7 +40,7 @@ def _compare(d, txs, cpml, mask, kind, field=None, gf_pair=None, path=("auto", 0
     dtype = np.float64 if kind == "f64" else np.float32
     prec = "ref" if kind == "ref" else "fast"
     f, g, cp = gpu_run(d, txs, cpml, mask, dtype, prec, field=field, gf=None if gf_pair is None else gf_pair[1],
-                       path=path[0], tile_y=path[1], block_z=path[2])
+                       path=path[0], tile_y=path[1], block_z=path[2], waves=path[3])
     tol = TOL[kind]
     errs = {
         "p_rec": rel_l2(g.rec[0], ref.tx["rec"][0]),
