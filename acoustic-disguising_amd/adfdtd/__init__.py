@@ -13,5 +13,6 @@ from .illumination import c_for_n, fibonacci_sphere, illumination_points, impuls
 from .extrapolation import PFromPin, PVFromPin, PVFromPV
 from .solver import Solver, dtype_code
 from .batch import Batch
+from .server import Server
 
 __all__ = [n for n in dir() if not n.startswith("_")]

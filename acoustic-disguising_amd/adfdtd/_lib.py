@@ -29,7 +29,12 @@ EXPORTS = (
     "adf_solver_last_step_ms", "adf_solver_set_option", "adf_solver_sweep_stats", "adf_solver_next_step",
     "adf_batch_create", "adf_batch_destroy", "adf_batch_add", "adf_batch_remove", "adf_batch_size",
     "adf_batch_run", "adf_batch_set_option", "adf_batch_last_step_ms",
+    "adf_server_create", "adf_server_destroy", "adf_server_set_option", "adf_server_submit", "adf_server_poll",
+    "adf_server_wait", "adf_server_get_trace", "adf_server_get_field", "adf_server_request_error",
+    "adf_server_release", "adf_server_stats",
 )
+ADF_REQ_QUEUED, ADF_REQ_RUNNING, ADF_REQ_DONE, ADF_REQ_FAILED = 0, 1, 2, -1
+ADF_SERVER_CHUNK, ADF_SERVER_BUDGET = 1, 2
 ADF_OPT_PATH, ADF_OPT_TILE_Y, ADF_OPT_BLOCK_Z, ADF_OPT_TIME_SWEEP, ADF_OPT_WAVES, ADF_OPT_BLOCK_Z_GEN = 1, 2, 3, 4, 5, 6
 ADF_BATCH_PERSISTENT, ADF_BATCH_GRAPH_STEPS = 1, 2
 
@@ -50,6 +55,29 @@ class AdfDomain(C.Structure):
 class AdfCpml(C.Structure):
     _fields_ = [("npml", C.c_int64)] + [(n, C.c_void_p) for n in
                                         ("a_l", "a_r", "a_hl", "a_hr", "b_l", "b_r", "b_hl", "b_hr")]
+
+
+class AdfRequest(C.Structure):
+    """include/adfdtd.h adf_request."""
+
+    _fields_ = [
+        ("dom", AdfDomain), ("dtype", C.c_int), ("cpml", AdfCpml), ("mask_shape", C.c_int),
+        ("mask_params", C.c_double * 4), ("xc", C.c_void_p), ("yc", C.c_void_p), ("zc", C.c_void_p),
+        ("mask4", C.c_void_p), ("ntx", C.c_int64), ("idx", C.c_void_p), ("w", C.c_void_p), ("src", C.c_void_p),
+        ("dir", C.c_void_p), ("vn_src", C.c_void_p), ("field", C.c_void_p), ("start_step", C.c_int64),
+        ("end_step", C.c_int64), ("want", C.c_uint32), ("keep_field", C.c_int),
+    ]
+
+
+class AdfServerInfo(C.Structure):
+    """include/adfdtd.h adf_server_info."""
+
+    _fields_ = [
+        ("completed", C.c_int64), ("failed", C.c_int64), ("running", C.c_int64), ("queued", C.c_int64),
+        ("bytes_in_use", C.c_uint64), ("peak_bytes", C.c_uint64), ("budget_bytes", C.c_uint64),
+        ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("solves_per_s", C.c_double),
+        ("busy_s", C.c_double), ("glups", C.c_double),
+    ]
 
 
 class AdfError(RuntimeError):
@@ -104,6 +132,17 @@ def load():
             "adf_batch_run": ([p, i64], i32),
             "adf_batch_set_option": ([p, i32, i64], i32),
             "adf_batch_last_step_ms": ([p, C.POINTER(d)], i32),
+            "adf_server_create": ([i32, C.c_uint64, C.POINTER(p)], i32),
+            "adf_server_destroy": ([p], i32),
+            "adf_server_set_option": ([p, i32, i64], i32),
+            "adf_server_submit": ([p, C.POINTER(AdfRequest), C.POINTER(i64)], i32),
+            "adf_server_poll": ([p, i64, C.POINTER(i32)], i32),
+            "adf_server_wait": ([p, i64, d, C.POINTER(i32)], i32),
+            "adf_server_get_trace": ([p, i64, i32, p], i32),
+            "adf_server_get_field": ([p, i64, p], i32),
+            "adf_server_request_error": ([p, i64], C.c_char_p),
+            "adf_server_release": ([p, i64], i32),
+            "adf_server_stats": ([p, C.POINTER(AdfServerInfo)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -63,6 +63,7 @@ typedef struct adf_cpml {
 typedef struct adf_ctx adf_ctx;
 typedef struct adf_solver adf_solver;
 typedef struct adf_batch adf_batch;
+typedef struct adf_server adf_server;
 
 /* ---------------------------------------------------------------- context */
 
@@ -243,6 +244,88 @@ int adf_batch_run(adf_batch* b, int64_t nsteps);
 int adf_batch_set_option(adf_batch* b, int option, int64_t value);
 /* Mean device time per batched step of the last adf_batch_run (ms). */
 int adf_batch_last_step_ms(adf_batch* b, double* ms);
+
+/* ----------------------------------------------------------------- server
+ *
+ * Served form (an addition): a request loop on its own host thread and HIP
+ * context.  Requests (one run_fdtd! each: src/simulation.jl:542-557, with
+ * the set-up of src/greens.jl:151-164 or src/hologram.jl:215-248) are
+ * admitted in arrival order while their device footprint fits the budget,
+ * join the batch of their dtype, advance together with every other live
+ * request in chunks of steps, and are retired when they reach end_step:
+ * the requested traces are copied to host memory and the device memory is
+ * released.  A request's results are the same, value for value, as an
+ * adf_solver_run of the same inputs, whatever else is in flight.  No GF map
+ * (those solves run alone through adf_solver_run).  Latency is measured
+ * from submission to retirement. */
+typedef struct adf_request {
+    adf_domain dom;
+    int dtype;
+    adf_cpml cpml;
+    /* scatterer: 0 none, 1..4 shapes of adf_solver_build_mask (params, and
+     * the n coordinates xc/yc/zc of x(d), y(d), z(d)), -1 explicit Bool
+     * (nx,ny,nz,4) mask4 */
+    int mask_shape;
+    double mask_params[4];
+    const double* xc;
+    const double* yc;
+    const double* zc;
+    const uint8_t* mask4;
+    /* transceivers as adf_solver_set_transceivers */
+    int64_t ntx;
+    const int64_t* idx;
+    const void* w;
+    const void* src;
+    const void* dir;
+    const void* vn_src;
+    /* initial field (nx,ny,nz,4), NULL = zeros (src/simulation.jl:468-518
+     * initialField for holograms) */
+    const void* field;
+    int64_t start_step, end_step;
+    /* bitmask of trace buffers to return (bit b = buffer id b of
+     * adf_solver_get_trace); 0 = p, vx, vy, vz and vn receivers (0x1f) */
+    uint32_t want;
+    int keep_field; /* 1: also return the final field */
+} adf_request;
+
+#define ADF_REQ_QUEUED 0
+#define ADF_REQ_RUNNING 1
+#define ADF_REQ_DONE 2
+#define ADF_REQ_FAILED -1
+
+typedef struct adf_server_info {
+    int64_t completed, failed, running, queued;
+    uint64_t bytes_in_use, peak_bytes, budget_bytes;
+    double p50_ms, p99_ms, mean_ms; /* submission to retirement */
+    double solves_per_s;            /* completed / (last completion - first submission) */
+    double busy_s;                  /* time spent stepping */
+    double glups;                   /* cell updates per busy second / 1e9 */
+} adf_server_info;
+
+/* hbm_budget_bytes: device bytes live requests may hold (0 = 85 % of the
+ * free memory at creation). */
+int adf_server_create(int device, uint64_t hbm_budget_bytes, adf_server** out);
+/* Finishes running requests, fails queued ones, releases everything. */
+int adf_server_destroy(adf_server* sv);
+/*   ADF_SERVER_CHUNK   steps per scheduling round (default 64)
+ *   ADF_SERVER_BUDGET  device-memory budget in bytes */
+#define ADF_SERVER_CHUNK 1
+#define ADF_SERVER_BUDGET 2
+int adf_server_set_option(adf_server* sv, int option, int64_t value);
+/* Copies every input; returns at once with the request id. */
+int adf_server_submit(adf_server* sv, const adf_request* req, int64_t* id);
+int adf_server_poll(adf_server* sv, int64_t id, int* status);
+/* Block until done or failed, or timeout_s (< 0: no timeout). */
+int adf_server_wait(adf_server* sv, int64_t id, double timeout_s, int* status);
+/* Results of a done request: trace buffer `which` ((nt,ntx) as
+ * adf_solver_get_trace) and, with keep_field, the final field. */
+int adf_server_get_trace(adf_server* sv, int64_t id, int which, void* out);
+int adf_server_get_field(adf_server* sv, int64_t id, void* out);
+/* Message of a failed request ("" otherwise); valid until release. */
+const char* adf_server_request_error(adf_server* sv, int64_t id);
+/* Forget a done or failed request and free its host results. */
+int adf_server_release(adf_server* sv, int64_t id);
+int adf_server_stats(adf_server* sv, adf_server_info* st);
 
 #ifdef __cplusplus
 }

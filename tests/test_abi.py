@@ -39,3 +39,28 @@ def test_domain_struct_layout_matches_reference_field_order():
     names = [f[0] for f in L.AdfDomain._fields_]
     assert names == ["tmax", "xmax", "ymax", "zmax", "nx", "ny", "nz", "c0", "r0", "cf", "dx", "dy", "dz",
                      "fact_m0", "fact_m1", "dt", "nt", "z0"]
+
+
+def test_struct_layouts_match_the_c_header(tmp_path):
+    """sizeof/offsetof of every struct of include/adfdtd.h, compiled with gcc,
+    equal the ctypes mirrors."""
+    import subprocess
+
+    structs = {"adf_domain": L.AdfDomain, "adf_cpml": L.AdfCpml, "adf_request": L.AdfRequest,
+               "adf_server_info": L.AdfServerInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "adfdtd.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in cls._fields_:
+            lines.append(f'printf("{cname}.{f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for cname, cls in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f in cls._fields_:
+            assert int(got[f"{cname}.{f[0]}"]) == getattr(cls, f[0]).offset, (cname, f[0])
