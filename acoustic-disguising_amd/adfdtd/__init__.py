@@ -14,5 +14,6 @@ from .extrapolation import PFromPin, PVFromPin, PVFromPV
 from .solver import Solver, dtype_code
 from .batch import Batch
 from .server import Server
+from .autotune import tune_batch, tune_solver
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -81,6 +81,8 @@ class Solver:
                 "adf_solver_create")
         self.h = h
         self.ntx = 0
+        self._cpml = cpml
+        self._masked = False
 
     @classmethod
     def cached(cls, dom: Domain, code: int, cpml: Optional[Cpml], device: int = 0) -> "Solver":
@@ -152,10 +154,12 @@ class Solver:
     def set_mask(self, va) -> None:
         if va is None:
             L.check(self.lib.adf_solver_set_mask(self.h, None), "set_mask")
+            self._masked = False
             return
         d = self.dom
         m = self._arr(va, (4, d.nz, d.ny, d.nx), np.uint8)
         L.check(self.lib.adf_solver_set_mask(self.h, L.ptr(m)), "set_mask")
+        self._masked = True
 
     def build_mask(self, shape: str, params) -> None:
         """Scatterer mask built on the device (src/geometry.jl:86-280)."""
@@ -166,6 +170,7 @@ class Solver:
         xs = [np.ascontiguousarray(cs[a]["p"]) for a in ("x", "y", "z")]
         L.check(self.lib.adf_solver_build_mask(self.h, SHAPES[shape], L.ptr(prm), *[L.ptr(v) for v in xs]),
                 "build_mask")
+        self._masked = shape != "none"
 
     def get_mask(self) -> np.ndarray:
         d = self.dom

@@ -149,6 +149,15 @@ void loop(adf_server* S) {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty() || !S->live.empty(); });
             if (S->stop && S->live.empty() && S->queue.empty()) break;
+            if (S->live.empty() && !S->stop) {
+                // idle server: give a burst of submissions a moment to arrive
+                // so that it starts as one batch
+                size_t seen = 0;
+                for (int i = 0; i < 20 && S->queue.size() > seen; ++i) {
+                    seen = S->queue.size();
+                    S->cv_work.wait_for(lk, std::chrono::microseconds(500));
+                }
+            }
             if (S->stop) {  // drop what has not started
                 for (Request* r : S->queue) {
                     r->status = ADF_REQ_FAILED;

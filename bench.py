@@ -123,6 +123,7 @@ def gpu_leg(args):
         cp = A.make_cpml(d, p, 1e-4, FC, dtype)
         s = A.Solver(d, A.dtype_code(dtype, prec), cp)
         s.build_mask("sphere", [24 * d.dx])
+        shape = A.tune_solver(s)
         s.set_transceivers(A.transceivers_to_grid(d, txs, dtype))
         s.run(1, args.warmup)
         s.set_option(L.ADF_OPT_TIME_SWEEP, 1)
@@ -132,7 +133,7 @@ def gpu_leg(args):
         sweep_ms, launches = s.sweep_stats()
         out[name] = {"wall_s": wall, "steps": steps, "ms_per_step": wall / steps * 1e3,
                      "dev_ms_per_step": s.last_step_ms(), "sweep_ms": sweep_ms, "sweep_launches": launches,
-                     "glups": d.nx * d.ny * d.nz * steps / wall / 1e9,
+                     "glups": d.nx * d.ny * d.nz * steps / wall / 1e9, "tiling": shape,
                      "p_rec_peak": float(np.abs(s.get_trace(L.ADF_REC_P)[1:]).max())}
         del s
     return d, p, out
@@ -183,6 +184,139 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_each_way": n * 4, "ms": el * 1e3, "GBps_read_plus_write": 2 * n * 4 / el / 1e9}))
 
 
+def batch_bytes(specs):
+    return sum(algorithmic_bytes_per_step(sp["d"].nx, sp["npml"], 4) for sp in specs)
+
+
+def batched_leg(n_solves, sizes, nt, tune=True):
+    """One batch of impulsive-GF solves run to completion (nt steps each) in
+    one adf_batch_run; solves/s from the device time of the whole run."""
+    import adfdtd as A
+
+    specs = batch_specs(n_solves, sizes, nt)
+    sol = [make_batch_solver(sp) for sp in specs]
+    b = A.Batch(persistent=False, graph_steps=16)
+    shape = A.tune_batch(sol, batch=b) if tune else None
+    for x in sol:
+        b.add(x)
+    t0 = time.perf_counter()
+    b.run(nt)
+    wall = time.perf_counter() - t0
+    ms = b.last_step_ms()
+    run_s = ms * nt / 1e3
+    byts = batch_bytes(specs)
+    cells = sum(sp["d"].nx * sp["d"].ny * sp["d"].nz for sp in specs)
+    peak = float(np.abs(sol[0].get_trace(0)).max())
+    del b, sol
+    return {"solves": len(specs), "grids": sorted({sp["d"].nx for sp in specs}), "steps": nt,
+            "run_s": round(run_s, 4), "wall_s": round(wall, 4), "solves_per_s": round(len(specs) / run_s, 3),
+            "ms_per_batched_step": round(ms, 4), "glups": round(cells / (ms * 1e-3) / 1e9, 2),
+            "algorithmic_GBps": round(byts / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "tiling": shape,
+            "p_rec_peak_solve0": peak}
+
+
+def plain_solves_leg(n_solves, nt):
+    """The plain single-solve GPU path on the same 128^3 solves: one solver,
+    per solve reset + transceivers + nt steps + traces back, one after
+    another (impulsive_gfs' loop, src/greens.jl:143-171)."""
+    import adfdtd as A
+    from adfdtd import _lib as L
+
+    specs = batch_specs(n_solves, [128], nt)
+    d = specs[0]["d"]
+    s = A.Solver(d, A.dtype_code(np.float32, "fast"), A.make_cpml(d, specs[0]["npml"], 1e-4, FC, np.float32))
+    s.build_mask("sphere", [specs[0]["sphere"]])
+    A.tune_solver(s)
+    z = np.zeros(d.nt)
+    grids = []
+    for sp in specs:
+        txs = [A.make_transceiver(d, sp["src"], [1.0, 0.0, 0.0, 0.0], sp["tf"])]
+        txs += [A.make_transceiver(d, sp["rec"][i], np.r_[0.0, sp["rec_n"][i]], z) for i in range(len(sp["rec"]))]
+        grids.append(A.transceivers_to_grid(d, txs, np.float32))
+    s.reset()
+    s.set_transceivers(grids[0])
+    s.run(1, 20)  # warm-up
+    t0 = time.perf_counter()
+    dev = 0.0
+    for g in grids:
+        s.reset()
+        s.set_transceivers(g)
+        s.run(1, nt)
+        dev += s.last_step_ms() * nt / 1e3
+        s.get_trace(L.ADF_REC_P)
+        s.get_trace(L.ADF_VN_REC)
+    wall = time.perf_counter() - t0
+    return {"solves": n_solves, "steps": nt, "wall_s": round(wall, 4), "device_s": round(dev, 4),
+            "solves_per_s": round(n_solves / wall, 3), "solves_per_s_device": round(n_solves / dev, 3)}
+
+
+def served_specs(n_req, seed=5, sizes=(96, 128, 192, 256), steps=(800, 3000), rate=20.0):
+    """Poisson arrivals (rate per s, seeded) of impulsive-GF requests: grid
+    uniform over `sizes`, steps uniform in `steps`, scatterer uniform over
+    sphere/cube/ellipsoid with radius 10-30 % of the grid half-extent, a
+    monopole source on a Fibonacci sphere of radius 40 % of the half-extent,
+    100 receivers (p, vn) at 20 %."""
+    import adfdtd as A
+
+    rng = np.random.default_rng(seed)
+    dx = C_AIR / (2.5 * FC) / 10
+    t = 0.0
+    out = []
+    for i in range(n_req):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        n = int(rng.choice(sizes))
+        nt = int(rng.integers(steps[0], steps[1] + 1))
+        xmax = (n - 1) * dx / 2
+        d0 = A.make_domain(1.0, xmax, xmax, xmax, n, n, n, C_AIR, RHO_AIR, 0.5)
+        d = A.make_domain((nt - 0.5) * d0.dt, xmax, xmax, xmax, n, n, n, C_AIR, RHO_AIR, 0.5)
+        shape = str(rng.choice(["sphere", "cube", "ellipsoid"]))
+        r = rng.uniform(0.1, 0.3, 3) * xmax
+        params = {"sphere": [r[0]], "cube": [2 * r[0]], "ellipsoid": list(r)}[shape]
+        srcs, _ = A.fibonacci_sphere(16, 0.4 * xmax)
+        rec, rn = A.fibonacci_sphere(100, 0.2 * xmax)
+        tf, _ = A.impulsive_wavelet(d, FC)
+        out.append({"t": t, "d": d, "npml": 10, "shape": shape, "params": params, "src": srcs[i % 16],
+                    "tf": tf, "rec": rec, "rec_n": rn})
+    return out
+
+
+def served_leg(reqs, budget=200 << 30):
+    """Requests submitted at their arrival times (key "t", seconds) to a
+    running adf_server; latency is submission to retirement."""
+    import adfdtd as A
+
+    n_req = len(reqs)
+    prepared = []
+    for q in reqs:
+        d = q["d"]
+        z = np.zeros(d.nt)
+        txs = [A.make_transceiver(d, q["src"], [1.0, 0.0, 0.0, 0.0], q["tf"])]
+        txs += [A.make_transceiver(d, q["rec"][i], np.r_[0.0, q["rec_n"][i]], z) for i in range(len(q["rec"]))]
+        prepared.append((q, A.make_cpml(d, q["npml"], 1e-4, FC, np.float32),
+                         A.transceivers_to_grid(d, txs, np.float32)))
+    code = A.dtype_code(np.float32, "fast")
+    lups = sum(q["d"].nx ** 3 * q["d"].nt for q in reqs)
+    with A.Server(budget_bytes=budget, chunk=64) as sv:
+        t0 = time.perf_counter()
+        ids = []
+        for q, cp, g in prepared:
+            dt = q["t"] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(sv.submit(q["d"], code, cp, g, mask=(q["shape"], q["params"]), want=0x11))
+        for rid in ids:
+            sv.wait(rid, 600.0)
+        wall = time.perf_counter() - t0
+        st = sv.stats()
+    return {"requests": n_req, "grids": sorted({q["d"].nx for q in reqs}),
+            "steps": [min(q["d"].nt for q in reqs), max(q["d"].nt for q in reqs)],
+            "budget_GB": budget / 2 ** 30, "wall_s": round(wall, 3), "solves_per_s": round(n_req / wall, 3),
+            "p50_ms": round(st["p50_ms"], 1), "p99_ms": round(st["p99_ms"], 1), "mean_ms": round(st["mean_ms"], 1),
+            "glups_busy": round(st["glups"], 2), "glups_wall": round(lups / wall / 1e9, 2),
+            "peak_GB": round(st["peak_bytes"] / 2 ** 30, 3), "failed": st["failed"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +326,11 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-served", action="store_true")
+    ap.add_argument("--batch-solves", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=1200)
+    ap.add_argument("--served-requests", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -202,7 +341,7 @@ def main():
         return
     if args.profile_steps:
         args.steps, args.warmup, args.fp64_steps = args.profile_steps, 2, 0
-        args.skip_cpu = True
+        args.skip_cpu = args.skip_batch = args.skip_served = True
 
     d, p, g = gpu_leg(args)
     n = d.nx
@@ -230,13 +369,34 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_fused (field-update sweep)", "algorithmic_bytes_per_launch": b32,
-                     "avg_launch_ms": round(f32["sweep_ms"], 5), "launches": f32["sweep_launches"]},
+                     "avg_launch_ms": round(f32["sweep_ms"], 5), "launches": f32["sweep_launches"],
+                     "tiling": f32["tiling"]},
     }
     if "f64" in g:
         res["f64"] = {"glups": round(g["f64"]["glups"], 3), "ms_per_step": round(g["f64"]["ms_per_step"], 5),
                       "steps": g["f64"]["steps"], "sweep_ms": round(g["f64"]["sweep_ms"], 5),
                       "hbm_frac_sweep": round(algorithmic_bytes_per_step(n, p, 8) / (g["f64"]["sweep_ms"] * 1e-3)
                                               / 1e9 / HBM_PEAK_GBS, 4)}
+    if not args.skip_batch:
+        nb, ns = args.batch_solves, args.batch_steps
+        bt = batched_leg(nb, [128], ns)
+        un = batched_leg(48, [96, 128, 160], ns)
+        pl = plain_solves_leg(4, ns)
+        res["batched"] = {"workload": f"{nb} impulsive-GF solves, 128^3 FP32, 10-cell CPML, sphere r=16 cells, "
+                                      f"sources on a 40-cell Fibonacci sphere, 100 receivers at 20 cells, {ns} steps",
+                          **bt, "plain_path": pl,
+                          "speedup_vs_plain": round(bt["solves_per_s"] / pl["solves_per_s"], 2)}
+        res["batched_uneven"] = {"workload": f"48 solves, 16 each at 96^3/128^3/160^3 (radii scaled by n/128), "
+                                             f"{ns} steps", **un}
+    if not args.skip_served:
+        res["served"] = {"workload": "seeded Poisson arrivals (20 req/s) of FP32 impulsive-GF requests: grid "
+                                     "96/128/192/256^3, 800-3000 steps, sphere/cube/ellipsoid r=10-30 % of the "
+                                     "half-extent, 100 receivers, 200 GB budget",
+                         **served_leg(served_specs(args.served_requests))}
+        b128 = [dict(sp, t=0.0, shape="sphere", params=[sp["sphere"]])
+                for sp in batch_specs(args.batch_solves, [128], args.batch_steps)]
+        res["served_128"] = {"workload": f"the {args.batch_solves} solves of the 128^3 batched workload as requests, "
+                                         "all submitted at t=0", **served_leg(b128)}
     if not args.skip_cpu:
         d2, txs, p2 = build_workload(nt=4000)
         res["cpu_baseline"] = cpu_leg(d2, txs, p2)

@@ -64,3 +64,14 @@ def test_struct_layouts_match_the_c_header(tmp_path):
         assert int(got[cname]) == ctypes.sizeof(cls), cname
         for f in cls._fields_:
             assert int(got[f"{cname}.{f[0]}"]) == getattr(cls, f[0]).offset, (cname, f[0])
+
+
+def test_autotune_cache_roundtrip(tmp_path):
+    from adfdtd import autotune
+
+    path = str(tmp_path / "sub" / "tune.json")
+    assert autotune._load(path) == {}
+    autotune._store(path, "solver:1:8x8x8:p2:m0", {"tile_y": 2, "block_z": 8})
+    autotune._store(path, "batch:x", {"persistent": 1})
+    data = autotune._load(path)
+    assert data["solver:1:8x8x8:p2:m0"]["block_z"] == 8 and data["batch:x"]["persistent"] == 1
