@@ -152,10 +152,14 @@ void loop(adf_server* S) {
             if (S->live.empty() && !S->stop) {
                 // idle server: give a burst of submissions a moment to arrive
                 // so that it starts as one batch
-                size_t seen = 0;
-                for (int i = 0; i < 20 && S->queue.size() > seen; ++i) {
-                    seen = S->queue.size();
-                    S->cv_work.wait_for(lk, std::chrono::microseconds(500));
+                // (until 1 ms passes without a new one, at most 20 ms)
+                using clk = std::chrono::steady_clock;
+                const auto t_end = clk::now() + std::chrono::milliseconds(20);
+                for (;;) {
+                    const size_t seen = S->queue.size();
+                    const auto until = std::min(clk::now() + std::chrono::milliseconds(1), t_end);
+                    if (!S->cv_work.wait_until(lk, until, [&] { return S->stop || S->queue.size() > seen; })) break;
+                    if (S->stop || clk::now() >= t_end) break;
                 }
             }
             if (S->stop) {  // drop what has not started

@@ -348,7 +348,7 @@ def main():
     f32 = g["f32"]
     b32 = algorithmic_bytes_per_step(n, p, 4)
     achieved = b32 / (f32["sweep_ms"] * 1e-3) / 1e9
-    traffic = read_traffic("fused_sweep_256_f32_pmc.txt")
+    traffic = read_traffic("sweep_256_f32.md")
     res = {
         "metric": "GLUPS (256^3 FDTD, one step = one forward_onestep!)",
         "value": round(f32["glups"], 3),
@@ -368,7 +368,7 @@ def main():
         "hbm_frac_step": round(b32 / (f32["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_fused (field-update sweep)", "algorithmic_bytes_per_launch": b32,
+                     "kernel": "k_sweep general + interior tiles (field-update sweep)", "algorithmic_bytes_per_launch": b32,
                      "avg_launch_ms": round(f32["sweep_ms"], 5), "launches": f32["sweep_launches"],
                      "tiling": f32["tiling"]},
     }
