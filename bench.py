@@ -131,10 +131,26 @@ def gpu_leg(args):
         s.run(args.warmup + 1, args.warmup + steps)
         wall = time.perf_counter() - t0
         sweep_ms, launches = s.sweep_stats()
+        peak = float(np.abs(s.get_trace(L.ADF_REC_P)[1:]).max())
+        # host-buffer boundary: what a run_fdtd! call adds around the steps
+        # (field, CPML memory and transceivers in; field, CPML memory and all
+        # ten trace buffers out)
+        g = A.transceivers_to_grid(d, txs, dtype)
+        f = np.zeros((4, d.nz, d.ny, d.nx), dtype)
+        t1 = time.perf_counter()
+        s.set_field(f)
+        s.set_cpml_memory(cp)
+        s.set_transceivers(g)
+        t2 = time.perf_counter()
+        s.get_field(f)
+        s.get_cpml_memory(cp)
+        s.get_transceivers(g)
+        t3 = time.perf_counter()
+        out[name + "_pcie"] = {"upload_s": t2 - t1, "download_s": t3 - t2}
         out[name] = {"wall_s": wall, "steps": steps, "ms_per_step": wall / steps * 1e3,
                      "dev_ms_per_step": s.last_step_ms(), "sweep_ms": sweep_ms, "sweep_launches": launches,
                      "glups": d.nx * d.ny * d.nz * steps / wall / 1e9, "tiling": shape,
-                     "p_rec_peak": float(np.abs(s.get_trace(L.ADF_REC_P)[1:]).max())}
+                     "p_rec_peak": peak}
         del s
     return d, p, out
 
@@ -372,6 +388,12 @@ def main():
                      "avg_launch_ms": round(f32["sweep_ms"], 5), "launches": f32["sweep_launches"],
                      "tiling": f32["tiling"]},
     }
+    io = g["f32_pcie"]
+    res["pcie_inclusive"] = {
+        "glups": round(n ** 3 * args.steps / (f32["wall_s"] + io["upload_s"] + io["download_s"]) / 1e9, 3),
+        "upload_s": round(io["upload_s"], 4), "download_s": round(io["download_s"], 4),
+        "note": "value plus one host-to-device upload of field, CPML memory and transceivers and one download "
+                "of field, CPML memory and the ten trace buffers (a run_fdtd! call through the host boundary)"}
     if "f64" in g:
         res["f64"] = {"glups": round(g["f64"]["glups"], 3), "ms_per_step": round(g["f64"]["ms_per_step"], 5),
                       "steps": g["f64"]["steps"], "sweep_ms": round(g["f64"]["sweep_ms"], 5),
