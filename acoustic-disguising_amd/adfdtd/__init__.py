@@ -15,5 +15,6 @@ from .solver import Solver, dtype_code
 from .batch import Batch
 from .server import Server
 from .autotune import tune_batch, tune_solver
+from .kh import kh_analytic_accumulate
 
 __all__ = [n for n in dir() if not n.startswith("_")]

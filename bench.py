@@ -267,6 +267,39 @@ def plain_solves_leg(n_solves, nt):
             "solves_per_s": round(n_solves / wall, 3), "solves_per_s_device": round(n_solves / dev, 3)}
 
 
+def kh_leg(which="p_v"):
+    """Post-hoc KH of the 256^3 workload's surfaces: the 2000-point KH sphere
+    (r = 40 cells) extrapolated to the 500 far-field points (r = 80 cells)
+    over 2000 steps, analytic kernel evaluated on the device (no dense K:
+    it would be 8 GB).  The work is the arrival-window MACs; VALU-bound."""
+    import adfdtd as A
+
+    d, _, _ = build_workload(nt=2000)
+    op, on = A.fibonacci_sphere(2000, 40 * d.dx)
+    ip, inn = A.fibonacci_sphere(500, 80 * d.dx)
+    sigma = math.sqrt(math.log(2)) / (2 * math.pi * FC)
+    nt = 2000
+    tg = np.arange(nt) * d.dt
+    inp = np.random.default_rng(7).standard_normal((2000, nt))
+    out = np.zeros((500, nt))
+    A.kh_analytic_accumulate(out, which, op, on, ip, inn, tg, sigma, C_AIR, RHO_AIR, inp[:, :nt], dtype=np.float32)
+    t0 = time.perf_counter()
+    A.kh_analytic_accumulate(out, which, op, on, ip, inn, tg, sigma, C_AIR, RHO_AIR, inp, dtype=np.float32)
+    wall = time.perf_counter() - t0
+    ms = A.kh.kh_last_ms()
+    # window MACs: per (r, s) pair, sum over t of the window samples <= t
+    r = np.linalg.norm(ip[None, :, :] - op[:, None, :], axis=2) / C_AIR
+    lo = np.clip(np.floor((r - 10 * sigma) / d.dt), 0, nt)
+    hi = np.clip(np.floor((r + 10 * sigma) / d.dt) + 1, 0, nt)
+    macs = float(np.sum((hi - lo) * (nt - (lo + hi) / 2)))
+    dense = 500 * 2000 * nt * (nt + 1) / 2
+    tflops = 2 * macs / (ms * 1e-3) / 1e12
+    return {"kernel": which, "n_outer": 2000, "n_inner": 500, "nt": nt, "device_ms": round(ms, 3),
+            "wall_s_incl_transfers": round(wall, 4), "window_macs": macs, "dense_equivalent_macs": dense,
+            "tflops_window": round(tflops, 2), "frac_fp32_valu_peak": round(tflops / 157.3, 4),
+            "dense_equivalent_tmacs_per_s": round(dense / (ms * 1e-3) / 1e12, 1)}
+
+
 def served_specs(n_req, seed=5, sizes=(96, 128, 192, 256), steps=(800, 3000), rate=20.0):
     """Poisson arrivals (rate per s, seeded) of impulsive-GF requests: grid
     uniform over `sizes`, steps uniform in `steps`, scatterer uniform over
@@ -410,6 +443,9 @@ def main():
                           "speedup_vs_plain": round(bt["solves_per_s"] / pl["solves_per_s"], 2)}
         res["batched_uneven"] = {"workload": f"48 solves, 16 each at 96^3/128^3/160^3 (radii scaled by n/128), "
                                              f"{ns} steps", **un}
+    if not args.skip_batch:
+        res["kh_post_hoc"] = {"workload": "analytic KH p_v kernel, 2000-point KH sphere -> 500 far-field points, "
+                                          "2000 steps, FP32 kernel samples and inputs, double sums", **kh_leg()}
     if not args.skip_served:
         res["served"] = {"workload": "seeded Poisson arrivals (20 req/s) of FP32 impulsive-GF requests: grid "
                                      "96/128/192/256^3, 800-3000 steps, sphere/cube/ellipsoid r=10-30 % of the "

@@ -245,6 +245,31 @@ int adf_batch_set_option(adf_batch* b, int option, int64_t value);
 /* Mean device time per batched step of the last adf_batch_run (ms). */
 int adf_batch_last_step_ms(adf_batch* b, double* ms);
 
+/* --------------------------------------------------------- post-hoc KH
+ *
+ * Analytic Kirchhoff-Helmholtz kernels evaluated on the device and
+ * convolved with surface traces, replacing eval_one_kernel
+ * (src/kh_kernels.jl:106-157) + eval_kernel_pedestal (:174-203) +
+ * direct_kh_accumulate! (:256-296) without forming the dense kernel:
+ *   out[t,s] += alpha * sum_r sum_{tt=1}^{min(t,ntK)} K(tt;r,s) in[t-tt+1,r]
+ * which: 0 p_p, 1 p_v, 2 v_p, 3 v_v, at t_grid[tt] (ntK increasing times)
+ * with Gaussian width sigma_t, sound speed c0, density rho0.  pedestal = 1
+ * (v_v): K - K_dc plus alpha K_dc[r,s] cumsum(in)[t,r], as the reference's
+ * truncation-friendly form.  Points and normals are the Julia (n,3)
+ * matrices (column-major); in (nt, n_outer) and out (nt, n_inner) are
+ * column-major host doubles, out accumulated.  dtype ADF_F64: kernel values
+ * and products in double; ADF_F32 / ADF_F32_REF: kernel values rounded to
+ * Float32 as eval_one_kernel(; T = Float32), inputs rounded to Float32,
+ * Float32 products summed in double.  Synchronous. */
+int adf_kh_analytic_accumulate(adf_ctx* ctx, int which, int dtype, int64_t n_outer, const double* outer_pts,
+                               const double* outer_nrm, int64_t n_inner, const double* inner_pts,
+                               const double* inner_nrm, const double* t_grid, int64_t ntK, double sigma_t,
+                               double c0, double rho0, int pedestal, double alpha, int64_t nt, const double* in,
+                               double* out);
+/* Device time (ms) of the prefix-sum and convolution kernels of this
+ * thread's last adf_kh_analytic_accumulate (measurement support). */
+int adf_kh_last_ms(double* ms);
+
 /* ----------------------------------------------------------------- server
  *
  * Served form (an addition): a request loop on its own host thread and HIP
