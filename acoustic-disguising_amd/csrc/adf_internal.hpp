@@ -93,6 +93,8 @@ struct adf_solver {
     int block_z = 0;   // 0 auto
     int waves = 0;     // waves per workgroup of the fused kernel (0 auto)
     int block_z_gen = 0;  // planes per general tile (0 auto)
+    int pipeline = 0;     // interior tiles: prefetch the next plane
+    int concurrent = 1;   // general and interior launches on two streams
 
     // GF extrapolation map
     int gf_kind = 0;

@@ -454,6 +454,12 @@ int solver_set_option(adf_solver* s, int option, int64_t value) {
     } else if (option == ADF_OPT_WAVES) {
         if (value != 0 && value != 1 && value != 2 && value != 4) return fail(ADF_EINVAL, "ADF_OPT_WAVES must be 0, 1, 2 or 4");
         s->waves = (int)value;
+    } else if (option == ADF_OPT_PIPELINE) {
+        if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_PIPELINE must be 0 or 1");
+        s->pipeline = (int)value;
+    } else if (option == ADF_OPT_CONCURRENT) {
+        if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_CONCURRENT must be 0 or 1");
+        s->concurrent = (int)value;
     } else {
         return fail(ADF_EINVAL, "unknown option");
     }
@@ -716,7 +722,7 @@ template <typename T, bool EXACT, int CPL, int TY>
 static void launch_fused_ty(adf_solver* s, const Fused<T>& F, hipStream_t st) {
     const int W = s->waves ? s->waves : 1;
     hipStream_t side = s->ctx->side;
-    const bool fork = s->n_gen && s->n_int;
+    const bool fork = s->n_gen && s->n_int && s->concurrent;
     if (fork) {
         (void)hipEventRecord(s->ev_fork, st);
         (void)hipStreamWaitEvent(side, s->ev_fork, 0);
@@ -724,9 +730,15 @@ static void launch_fused_ty(adf_solver* s, const Fused<T>& F, hipStream_t st) {
     if (s->n_gen)
         hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, TY, true>), dim3((unsigned)((s->n_gen + W - 1) / W)), dim3(64 * W), 0,
                            fork ? side : st, F, (const uint2*)s->tiles_gen, s->n_gen);
-    if (s->n_int)
-        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, TY, false>), dim3((unsigned)((s->n_int + W - 1) / W)), dim3(64 * W), 0,
-                           st, F, (const uint2*)s->tiles_int, s->n_int);
+    if (s->n_int) {
+        const dim3 g((unsigned)((s->n_int + W - 1) / W));
+        if (s->pipeline)
+            hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, TY, false, true>), g, dim3(64 * W), 0, st, F,
+                               (const uint2*)s->tiles_int, s->n_int);
+        else
+            hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, TY, false>), g, dim3(64 * W), 0, st, F,
+                               (const uint2*)s->tiles_int, s->n_int);
+    }
     if (fork) {
         (void)hipEventRecord(s->ev_join, side);
         (void)hipStreamWaitEvent(st, s->ev_join, 0);

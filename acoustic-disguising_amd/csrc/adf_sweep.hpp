@@ -239,8 +239,9 @@ __device__ __forceinline__ void sweep_int_tile(const Fused<T>& F, int tyi, int k
 
 // One solve's tiles (entries as xcd_tile describes), GEN: general tiles
 // (CPML slabs in y or z, boundary planes, clamped rows), else interior
-// tiles; W waves per workgroup take W consecutive tiles.
-template <typename T, bool EXACT, int CPL, int TY, bool GEN>
+// tiles; W waves per workgroup take W consecutive tiles.  PF: interior
+// tiles load plane k+1 before computing plane k.
+template <typename T, bool EXACT, int CPL, int TY, bool GEN, bool PF = false>
 __global__ __launch_bounds__(256) void k_sweep(Fused<T> F, const uint2* __restrict__ tiles, int ntiles) {
     const int W = (int)(blockDim.x >> 6);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -250,7 +251,7 @@ __global__ __launch_bounds__(256) void k_sweep(Fused<T> F, const uint2* __restri
     const int tyi = (int)(e.x & 0xffffu), k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0xffffu);
     const bool masked = (e.x >> 31) != 0;
     if constexpr (GEN) fused_tile<T, EXACT, CPL, TY, true>(F, tyi, k0, k1, masked);
-    else sweep_int_tile<T, EXACT, CPL, TY>(F, tyi, k0, k1, masked);
+    else sweep_int_tile<T, EXACT, CPL, TY, PF>(F, tyi, k0, k1, masked);
 }
 
 }  // namespace adf

@@ -34,6 +34,12 @@ sys.path.insert(0, os.path.join(ROOT, "acoustic-disguising_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+T_START = time.perf_counter()
+
+
+def log(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.perf_counter() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
 C_AIR, RHO_AIR, FC = 343.0, 1.21, 2000.0
 
 
@@ -393,6 +399,7 @@ def main():
         args.skip_cpu = args.skip_batch = args.skip_served = True
 
     d, p, g = gpu_leg(args)
+    log("256^3 plain-path legs done")
     n = d.nx
     f32 = g["f32"]
     b32 = algorithmic_bytes_per_step(n, p, 4)
@@ -435,8 +442,11 @@ def main():
     if not args.skip_batch:
         nb, ns = args.batch_solves, args.batch_steps
         bt = batched_leg(nb, [128], ns)
+        log("batched 128^3 leg done")
         un = batched_leg(48, [96, 128, 160], ns)
+        log("batched uneven leg done")
         pl = plain_solves_leg(4, ns)
+        log("plain-path solves leg done")
         res["batched"] = {"workload": f"{nb} impulsive-GF solves, 128^3 FP32, 10-cell CPML, sphere r=16 cells, "
                                       f"sources on a 40-cell Fibonacci sphere, 100 receivers at 20 cells, {ns} steps",
                           **bt, "plain_path": pl,
@@ -446,11 +456,13 @@ def main():
     if not args.skip_batch:
         res["kh_post_hoc"] = {"workload": "analytic KH p_v kernel, 2000-point KH sphere -> 500 far-field points, "
                                           "2000 steps, FP32 kernel samples and inputs, double sums", **kh_leg()}
+        log("post-hoc KH leg done")
     if not args.skip_served:
         res["served"] = {"workload": "seeded Poisson arrivals (20 req/s) of FP32 impulsive-GF requests: grid "
                                      "96/128/192/256^3, 800-3000 steps, sphere/cube/ellipsoid r=10-30 % of the "
                                      "half-extent, 100 receivers, 200 GB budget",
                          **served_leg(served_specs(args.served_requests))}
+        log("served Poisson leg done")
         b128 = [dict(sp, t=0.0, shape="sphere", params=[sp["sphere"]])
                 for sp in batch_specs(args.batch_solves, [128], args.batch_steps)]
         res["served_128"] = {"workload": f"the {args.batch_solves} solves of the 128^3 batched workload as requests, "
@@ -458,6 +470,7 @@ def main():
     if not args.skip_cpu:
         d2, txs, p2 = build_workload(nt=4000)
         res["cpu_baseline"] = cpu_leg(d2, txs, p2)
+        log("CPU baseline leg done")
     print(json.dumps(res))
 
 

@@ -195,6 +195,14 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
 /*   ADF_OPT_BLOCK_Z_GEN  planes per tile of the fused kernel's general tiles
  *                        (CPML slabs in y/z, boundary planes): 0 auto, >= 1 */
 #define ADF_OPT_BLOCK_Z_GEN 6
+/*   ADF_OPT_PIPELINE     1: interior tiles of the fused kernel load the next
+ *                        plane while the current one is computed (two plane
+ *                        buffers in registers); 0: one plane at a time */
+#define ADF_OPT_PIPELINE 7
+/*   ADF_OPT_CONCURRENT   1 (default): general and interior tiles of the fused
+ *                        kernel run as two launches on two streams at the
+ *                        same time; 0: one after the other on one stream */
+#define ADF_OPT_CONCURRENT 8
 int adf_solver_set_option(adf_solver* s, int option, int64_t value);
 
 /* Mean device time (ms) of one field-update launch and the number of
