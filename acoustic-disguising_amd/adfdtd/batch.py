@@ -20,7 +20,7 @@ from .solver import Solver, context
 class Batch:
     """Wraps adf_batch_* (include/adfdtd.h)."""
 
-    def __init__(self, device: int = 0, persistent: bool = True, graph_steps: int = 16):
+    def __init__(self, device: int = 0, persistent: bool = False, graph_steps: int = 16):
         self.lib = L.load()
         h = C.c_void_p()
         L.check(self.lib.adf_batch_create(context(device), C.byref(h)), "adf_batch_create")

@@ -30,18 +30,15 @@ struct TxDev {
 // takes tiles from chunk x's queue (ctr[x]) and, when it is empty, from the
 // other chunks' queues in turn; the last wave to finish resets the queue
 // heads for the next launch.
-template <typename T, bool EXACT, int CPL, int TY, bool GEN>
+template <typename T, bool EXACT, int CPL, int TY, int LIST, bool PF>
 __global__ __launch_bounds__(256) void k_sweep_batch(const Fused<T>* __restrict__ fs, const int* __restrict__ cur,
                                                      const uint2* __restrict__ tiles, int ntiles, unsigned* ctr,
                                                      int persistent) {
     auto run = [&](int t) {
         const uint2 e = tiles[t];
-        const int s = (int)((e.x >> 16) & 0x7fffu);
+        const int s = (int)((e.x >> 16) & 0xfffu);
         const Fused<T> F = fs[2 * s + cur[s]];
-        const int tyi = (int)(e.x & 0xffffu), k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0xffffu);
-        const bool masked = (e.x >> 31) != 0;
-        if constexpr (GEN) fused_tile<T, EXACT, CPL, TY, true>(F, tyi, k0, k1, masked);
-        else sweep_int_tile<T, EXACT, CPL, TY>(F, tyi, k0, k1, masked);
+        sweep_tile<T, EXACT, CPL, TY, LIST, PF>(F, e);
     };
     if (!persistent) {
         const int W = (int)(blockDim.x >> 6);
@@ -197,7 +194,7 @@ __global__ void k_mirror_batch(const TxDev<T>* __restrict__ td, const int* __res
     }
 }
 
-__global__ void k_advance(int* it0s, int* cur, int n) {
+static __global__ void k_advance(int* it0s, int* cur, int n) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     it0s[s] += 1;

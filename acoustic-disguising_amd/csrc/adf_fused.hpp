@@ -326,7 +326,7 @@ struct FA {
 // indices (always in bounds) instead of guards; values of rows, lanes and
 // planes outside the grid are computed but never stored.
 template <typename T, bool EXACT, int CPL, int TY, bool GEN>
-__device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int k0, int k1, bool masked) {
+__device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, int k1, bool masked) {
     typedef FA<T, EXACT> M_;
     typedef typename M_::A A;
     const int nx = F.nx, ny = F.ny, nz = F.nz, p = F.p, qv = p + 1;
@@ -335,7 +335,6 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int k0, i
     const int i0 = lane * CPL;
     const bool lane_ok = i0 < nx;
     const int i0c = lane_ok ? i0 : nx - CPL;  // clamped lane start (loads only)
-    const int j0 = tyi * TY;
     const int ks = k0 > 0 ? k0 - 1 : k0;  // first plane (halo plane when k0 > 0)
     const uint32_t* Mk = masked ? F.mask : nullptr;  // tiles without closed cells skip the mask
     const int sh = i0c & 31;
@@ -584,9 +583,9 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int tyi, int k0, i
 // of the list (8 contiguous chunks, chunk x spanning [x q + min(x, r),
 // +q + (x < r)) for n = 8 q + r), so each XCD sweeps a contiguous run of
 // neighbouring tiles, which re-read each other's halo rows from its L2.
-// Tile entries: x = masked << 31 | member << 16 | y-tile, y = first plane
-// << 16 | planes; `masked` is set when a cell of the tile or its halo is
-// closed by the scatterer mask (other tiles skip the mask loads).
+// Tile entries: x = masked << 31 | kind << 28 | member << 16 | first row,
+// y = first plane << 16 | planes; `masked` is set when a cell of the tile or
+// its halo is closed by the scatterer mask (other tiles skip the mask loads).
 __device__ __forceinline__ int xcd_chunk_start(int x, int n) { return x * (n >> 3) + min(x, n & 7); }
 __device__ __forceinline__ int xcd_chunk_size(int x, int n) { return (n >> 3) + (x < (n & 7) ? 1 : 0); }
 __device__ __forceinline__ int xcd_tile(int b, int n) { return xcd_chunk_start(b & 7, n) + (b >> 3); }

@@ -79,12 +79,11 @@ struct adf_solver {
     uint32_t* fix_cells = nullptr;
     int n_fix = 0;
 
-    // fused-sweep tile lists for tiling (tl_ty, tl_kb): general tiles (CPML
-    // slabs in y/z, boundary planes) and interior tiles
+    // fused-sweep tile lists for tiling (tl_ty, tl_kb, tl_kbg): interior
+    // tiles and the side list (slab and general tiles; LIST_* order)
     int tl_ty = 0, tl_kb = 0, tl_kbg = 0;
-    uint2* tiles_gen = nullptr;
-    uint2* tiles_int = nullptr;
-    int n_gen = 0, n_int = 0;
+    uint2* tiles[2] = {nullptr, nullptr};
+    int ntl[2] = {0, 0};
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     // kernel path and tiling (adf_solver_set_option)
@@ -93,7 +92,7 @@ struct adf_solver {
     int block_z = 0;   // 0 auto
     int waves = 0;     // waves per workgroup of the fused kernel (0 auto)
     int block_z_gen = 0;  // planes per general tile (0 auto)
-    int pipeline = 0;     // interior tiles: prefetch the next plane
+    int pipeline = 1;     // pipelined (prefetching) tile routines
     int concurrent = 1;   // general and interior launches on two streams
 
     // GF extrapolation map
@@ -125,7 +124,7 @@ struct adf_batch {
     std::vector<adf_solver*> members;
     int dtype = 0;
     bool dirty = true;
-    bool persistent = true;
+    bool persistent = false;
     int graph_steps = 16;     // steps per captured graph (0: no graph)
     // device tables (rebuilt when dirty)
     void* fs = nullptr;       // Fused<T>[2 * n]: [2s + cur] = (old = cur, new = cur ^ 1)
@@ -133,10 +132,9 @@ struct adf_batch {
     int* it0 = nullptr;       // per member: 0-based step index of the next step
     int* cur = nullptr;       // per member: buffer holding the current field
     // one group per (cells per lane, rows per wave, waves per workgroup);
-    // tile entries x = member, y = tzi << 16 | tyi
     struct Group {
-        int cpl, ty, waves;
-        int n[2], grid[2];      // [0] general tiles, [1] interior tiles
+        int cpl, ty, waves, pf;  // pf: pipelined tile routines
+        int n[2], grid[2];      // per tile list (LIST_* order)
         uint2* tiles[2];
         unsigned* ctr[2];       // 8 per-XCD queue heads + a finished-wave count
     };

@@ -468,7 +468,7 @@ __device__ __forceinline__ bool inside_shape(int shape, const double* prm, doubl
 }
 
 // One thread per 32-cell word of the solid plane.
-__global__ void k_mask_solid(uint32_t* solid, int shape, const double* prm, const double* xc,
+static __global__ void k_mask_solid(uint32_t* solid, int shape, const double* prm, const double* xc,
                              const double* yc, const double* zc, int nx, int ny, int nz, int nwx) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t total = (size_t)nwx * ny * nz;
@@ -485,7 +485,7 @@ __global__ void k_mask_solid(uint32_t* solid, int shape, const double* prm, cons
 }
 
 // Four update planes from the solid plane (geometry.jl:257-272).
-__global__ void k_mask_planes(uint32_t* planes, const uint32_t* solid, int nx, int ny, int nz, int nwx) {
+static __global__ void k_mask_planes(uint32_t* planes, const uint32_t* solid, int nx, int ny, int nz, int nwx) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t pl = (size_t)nwx * ny * nz;
     if (t >= pl) return;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -14,6 +15,7 @@
 #include "adf_fused.hpp"
 #include "adf_sweep.hpp"
 #include "adf_batch.hpp"
+#include "adf_launch.hpp"
 
 namespace adf {
 
@@ -132,7 +134,7 @@ void solver_free(adf_solver* s) {
         dfree(s->fieldb[b]);
         dfree(s->psib[b]);
     }
-    dfree(s->coef); dfree(s->mask); dfree(s->tiles_gen); dfree(s->tiles_int);
+    dfree(s->coef); dfree(s->mask); for (auto& t : s->tiles) dfree(t);
     if (s->ev_fork) hipEventDestroy(s->ev_fork);
     if (s->ev_join) hipEventDestroy(s->ev_join);
     for (hipEvent_t e : s->sweep_ev) hipEventDestroy(e);
@@ -441,7 +443,8 @@ int solver_set_option(adf_solver* s, int option, int64_t value) {
         if (value == 2 && !fused_supported(s)) return fail(ADF_EINVAL, "fused path needs 2 <= nx <= 256");
         s->path = (int)value;
     } else if (option == ADF_OPT_TILE_Y) {
-        if (value != 0 && value != 2 && value != 4) return fail(ADF_EINVAL, "ADF_OPT_TILE_Y must be 0, 2 or 4");
+        if (value != 0 && value != SWEEP_TY)
+            return fail(ADF_EINVAL, "ADF_OPT_TILE_Y must be 0 or " + std::to_string(SWEEP_TY));
         s->tile_y = (int)value;
     } else if (option == ADF_OPT_TIME_SWEEP) {
         s->time_sweep = value != 0;
@@ -541,7 +544,7 @@ bool fused_supported(const adf_solver* s) { return fused_cpl(s) > 0; }
 // per general tile, chosen so the launch has several waves per CU while
 // halo work stays small; general tiles (slower per plane) get fewer planes.
 void pick_tiles(const adf_solver* s, int* ty, int* kb, int* kbg) {
-    int t = s->tile_y ? s->tile_y : 2;
+    int t = SWEEP_TY;
     int k = s->block_z;
     if (!k) {
         const int64_t ny = s->dom.ny, nz = s->dom.nz;
@@ -552,7 +555,7 @@ void pick_tiles(const adf_solver* s, int* ty, int* kb, int* kbg) {
     }
     *ty = t;
     *kb = k;
-    *kbg = s->block_z_gen ? s->block_z_gen : std::max(2, k / 4);
+    *kbg = s->block_z_gen ? s->block_z_gen : 2;
 }
 
 static constexpr int KZ = 8;
@@ -628,55 +631,91 @@ static int open_rows(const adf_solver* s, std::vector<uint8_t>& open) {
 }
 
 static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, uint32_t m, const std::vector<uint8_t>& open,
-                        std::vector<uint2>& gen, std::vector<uint2>& in) {
-    const int ny = (int)s->dom.ny, nz = (int)s->dom.nz, p = (int)s->p;
+                        std::vector<uint2> (&out)[TILE_LISTS]) {
+    const int ny = (int)s->dom.ny, nz = (int)s->dom.nz, p = (int)s->p, qv = p + 1;
+    const bool cpP = s->p > 0 && (s->has_mask || s->p > 1), cpV = qv > 1;
     const int tiles_y = (ny + ty - 1) / ty;
-    const int jlo = std::max(p + 1, 2), jhi = std::min(ny - p - 1, ny - 1);
-    const int klo = std::max(p + 1, 2), khi = std::min(nz - p - 1, nz - 1);
-    auto yint = [&](int t) { const int j0 = t * ty; return j0 - 1 >= jlo && j0 + ty - 1 < jhi; };
-    // interior planes: tiles [k0, k1) with k0 - 1 >= klo and k1 - 1 < khi
-    const int za = klo + 1, zb = khi;
+    kb = std::min(kb, 65535);
+    kbg = std::min(kbg, 65535);
+    // regular rows / planes: owned ones >= 2 (no mirror), the halo >= 1,
+    // and nothing at or past n-1 (rigid plane, edge)
+    auto rgen = [&](int t) { const int j0 = t * ty; return j0 < 2 || j0 + ty > ny - 1; };
+    auto in_slab = [&](int i, int n, bool on, int w) { return on && (i < w || i >= n - w); };
+    auto rslab = [&](int t) {
+        const int j0 = t * ty;
+        for (int j = j0 - 1; j < j0 + ty; ++j) {
+            if (in_slab(j, ny, cpP, p)) return true;
+            if (j >= j0 && in_slab(j, ny, cpV, qv)) return true;
+        }
+        return false;
+    };
+    auto zslab = [&](int k0, int k1) {
+        for (int k = k0 - 1; k < k1; ++k) {
+            if (in_slab(k, nz, cpP, p)) return true;
+            if (k >= k0 && in_slab(k, nz, cpV, qv)) return true;
+        }
+        return false;
+    };
     // a tile needs the mask if a row of it or of its halo (rows j0-1..j0+ty,
     // planes k0-1..k1) is not fully open
-    auto tile = [&](int t, int k0, int k1) {
+    // entry of the tile of rows [j0, j0+h), planes [k0, k1)
+    auto tile = [&](int j0, int h, int k0, int k1, uint32_t kind) {
         bool all_open = true;
         for (int k = std::max(k0 - 1, 0); k <= std::min(k1, nz - 1) && all_open; ++k)
-            for (int j = std::max(t * ty - 1, 0); j <= std::min(t * ty + ty, ny - 1); ++j)
+            for (int j = std::max(j0 - 1, 0); j <= std::min(j0 + h, ny - 1); ++j)
                 if (!open[(size_t)k * ny + j]) { all_open = false; break; }
-        return make_uint2((all_open ? 0u : 0x80000000u) | (m << 16) | (uint32_t)t,
+        return make_uint2((all_open ? 0u : 0x80000000u) | (kind << 28) | (m << 16) | (uint32_t)j0,
                           ((uint32_t)k0 << 16) | (uint32_t)(k1 - k0));
     };
-    auto blocks = [&](int a, int b, int w, std::vector<std::pair<int, int>>& out) {
+    // z ranges: general [0, 2), regular [2, nz-1) split at the z slabs
+    // (interior planes [zi0, zi1): halo and owned planes clear of both
+    // phases' slabs), general [nz-1, nz).  Interior tiles take kb planes,
+    // the side tiles (short dependency chains next to the long interior
+    // ones) kbg planes.
+    std::vector<std::pair<int, int>> zb, zs;
+    auto blocks = [&](std::vector<std::pair<int, int>>& out, int a, int b, int w) {
         for (int k = a; k < b; k += w) out.push_back({k, std::min(k + w, b)});
     };
-    std::vector<std::pair<int, int>> zall, zint, zlow, zhigh;
-    blocks(0, nz, kbg, zall);
-    if (za < zb) {
-        blocks(za, zb, kb, zint);
-        blocks(0, za, kbg, zlow);
-        blocks(zb, nz, kbg, zhigh);
+    const int ra = std::min(2, nz), rb = std::max(ra, nz - 1);
+    int zi0 = std::max(ra, cpP || cpV ? p + 1 : ra), zi1 = std::min(rb, nz - p - 1);
+    if (zi0 >= zi1) zi0 = zi1 = rb;
+    blocks(zb, 0, ra, kbg);
+    blocks(zb, ra, zi0, kbg);
+    blocks(zb, zi0, zi1, kb);
+    blocks(zb, zi1, rb, kbg);
+    blocks(zb, rb, nz, kbg);
+    blocks(zs, 0, nz, kbg);
+    // (k0, t, entry) in (z-block, y-tile) order, so neighbouring entries
+    // share halo rows
+    // Rows are classified in tiles of ty rows; side tiles are then cut into
+    // tiles of SWEEP_TY_SIDE rows (a regular tile's row subsets are regular,
+    // and the slab routines are exact for rows outside the slab).
+    std::vector<std::tuple<int, int, uint2>> lists[TILE_LISTS];
+    for (int t = 0; t < tiles_y; ++t) {
+        const bool yg = rgen(t), ys = !yg && rslab(t);
+        const int j0 = t * ty, j1 = std::min(j0 + ty, ny);
+        for (auto& z : (yg || ys) ? zs : zb) {
+            uint32_t kind = TILE_GEN;
+            if (!yg && z.first >= 2 && z.second <= nz - 1) {
+                const bool zl = zslab(z.first, z.second);
+                kind = ys ? (zl ? TILE_SYZ : TILE_SY) : (zl ? TILE_SZ : TILE_INT);
+            }
+            if (kind == TILE_INT) {
+                lists[LIST_INT].emplace_back(z.first, j0, tile(j0, ty, z.first, z.second, kind));
+                continue;
+            }
+            for (int j = j0; j < j1; j += SWEEP_TY_SIDE)
+                lists[LIST_SIDE].emplace_back(z.first, j, tile(j, SWEEP_TY_SIDE, z.first, z.second, kind));
+        }
     }
-    gen.clear();
-    in.clear();
-    // general z-blocks below the interior, then the interior band (general
-    // rows and interior rows), then the blocks above
-    std::vector<std::pair<int, int>> zgen_band;
-    if (za < zb) blocks(za, zb, kbg, zgen_band);
-    if (za >= zb) {
-        for (auto& z : zall)
-            for (int t = 0; t < tiles_y; ++t) gen.push_back(tile(t, z.first, z.second));
-        return;
+    auto order = [](const std::tuple<int, int, uint2>& a, const std::tuple<int, int, uint2>& b) {
+        return std::get<0>(a) != std::get<0>(b) ? std::get<0>(a) < std::get<0>(b) : std::get<1>(a) < std::get<1>(b);
+    };
+    for (int c = 0; c < TILE_LISTS; ++c) {
+        std::sort(lists[c].begin(), lists[c].end(), order);
+        out[c].clear();
+        for (auto& e : lists[c]) out[c].push_back(std::get<2>(e));
     }
-    for (auto& z : zlow)
-        for (int t = 0; t < tiles_y; ++t) gen.push_back(tile(t, z.first, z.second));
-    for (auto& z : zgen_band)
-        for (int t = 0; t < tiles_y; ++t)
-            if (!yint(t)) gen.push_back(tile(t, z.first, z.second));
-    for (auto& z : zhigh)
-        for (int t = 0; t < tiles_y; ++t) gen.push_back(tile(t, z.first, z.second));
-    for (auto& z : zint)
-        for (int t = 0; t < tiles_y; ++t)
-            if (yint(t)) in.push_back(tile(t, z.first, z.second));
 }
 
 static int upload_tiles(const std::vector<uint2>& v, uint2** dst, int* n) {
@@ -691,21 +730,22 @@ static int upload_tiles(const std::vector<uint2>& v, uint2** dst, int* n) {
 }
 
 static int check_tiling(const adf_solver* s, int ty) {
-    if ((s->dom.ny + ty - 1) / ty > 65535 || s->dom.nz > 65535)
-        return fail(ADF_EINVAL, "grid too large for the fused sweep's tile list (ny/tile_y and nz must be < 65536)");
+    (void)ty;
+    if (s->dom.ny > 65535 || s->dom.nz > 65535)
+        return fail(ADF_EINVAL, "grid too large for the fused sweep's tile list (ny and nz must be < 65536)");
     return ADF_OK;
 }
 
 static int prepare_tiles(adf_solver* s, int ty, int kb, int kbg) {
-    if (s->tl_ty == ty && s->tl_kb == kb && s->tl_kbg == kbg && (s->tiles_gen || s->tiles_int)) return ADF_OK;
+    if (s->tl_ty == ty && s->tl_kb == kb && s->tl_kbg == kbg) return ADF_OK;
     int rc = check_tiling(s, ty);
     if (rc) return rc;
     std::vector<uint8_t> open;
     if ((rc = open_rows(s, open))) return rc;
-    std::vector<uint2> gen, in;
-    split_tiles(s, ty, kb, kbg, 0, open, gen, in);
-    if ((rc = upload_tiles(gen, &s->tiles_gen, &s->n_gen))) return rc;
-    if ((rc = upload_tiles(in, &s->tiles_int, &s->n_int))) return rc;
+    std::vector<uint2> lists[TILE_LISTS];
+    split_tiles(s, ty, kb, kbg, 0, open, lists);
+    for (int c = 0; c < TILE_LISTS; ++c)
+        if ((rc = upload_tiles(lists[c], &s->tiles[c], &s->ntl[c]))) return rc;
     if (!s->ev_fork) {
         HIPCHK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
@@ -714,48 +754,6 @@ static int prepare_tiles(adf_solver* s, int ty, int kb, int kbg) {
     s->tl_kb = kb;
     s->tl_kbg = kbg;
     return ADF_OK;
-}
-
-// General tiles on the side stream, interior tiles on the main stream, at
-// the same time (fork/join through events; both streams default priority).
-template <typename T, bool EXACT, int CPL, int TY>
-static void launch_fused_ty(adf_solver* s, const Fused<T>& F, hipStream_t st) {
-    const int W = s->waves ? s->waves : 1;
-    hipStream_t side = s->ctx->side;
-    const bool fork = s->n_gen && s->n_int && s->concurrent;
-    if (fork) {
-        (void)hipEventRecord(s->ev_fork, st);
-        (void)hipStreamWaitEvent(side, s->ev_fork, 0);
-    }
-    if (s->n_gen)
-        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, TY, true>), dim3((unsigned)((s->n_gen + W - 1) / W)), dim3(64 * W), 0,
-                           fork ? side : st, F, (const uint2*)s->tiles_gen, s->n_gen);
-    if (s->n_int) {
-        const dim3 g((unsigned)((s->n_int + W - 1) / W));
-        if (s->pipeline)
-            hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, TY, false, true>), g, dim3(64 * W), 0, st, F,
-                               (const uint2*)s->tiles_int, s->n_int);
-        else
-            hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, TY, false>), g, dim3(64 * W), 0, st, F,
-                               (const uint2*)s->tiles_int, s->n_int);
-    }
-    if (fork) {
-        (void)hipEventRecord(s->ev_join, side);
-        (void)hipStreamWaitEvent(st, s->ev_join, 0);
-    }
-}
-
-template <typename T, bool EXACT, int CPL>
-static void launch_fused_cpl(adf_solver* s, const Fused<T>& F, hipStream_t st) {
-    if (F.ty == 4) launch_fused_ty<T, EXACT, CPL, 4>(s, F, st);
-    else launch_fused_ty<T, EXACT, CPL, 2>(s, F, st);
-}
-
-template <typename T, bool EXACT>
-static void launch_fused(adf_solver* s, const Fused<T>& F, int cpl, hipStream_t st) {
-    if (cpl == 4) launch_fused_cpl<T, EXACT, 4>(s, F, st);
-    else if (cpl == 2) launch_fused_cpl<T, EXACT, 2>(s, F, st);
-    else launch_fused_cpl<T, EXACT, 1>(s, F, st);
 }
 
 // Start/stop events around one field-update launch (ADF_OPT_TIME_SWEEP).
@@ -954,7 +952,7 @@ static void batch_drop_tables(adf_batch* b) {
     b->txp = b->txv = nullptr;
     b->fix = nullptr;
     for (auto& g : b->groups)
-        for (int h = 0; h < 2; ++h) { dfree(g.tiles[h]); dfree(g.ctr[h]); }
+        for (int h = 0; h < TILE_LISTS; ++h) { dfree(g.tiles[h]); dfree(g.ctr[h]); }
     b->groups.clear();
     b->txp_lev.clear();
     b->txv_lev.clear();
@@ -1003,13 +1001,13 @@ static int batch_build(adf_batch* b) {
     std::vector<Fused<T>> fs(2 * n);
     std::vector<TxDev<T>> td(n);
     std::vector<int> it0(n), cur(n);
-    std::map<std::tuple<int, int, int>, std::vector<uint2>> gen_tiles, int_tiles;
+    std::map<std::tuple<int, int, int, int>, std::array<std::vector<uint2>, TILE_LISTS>> glists;
     // total work decides the z-block: enough tiles for every CU
     int64_t cells = 0;
     for (adf_solver* s : b->members) cells += s->n3;
     for (int m = 0; m < n; ++m) {
         adf_solver* s = b->members[m];
-        int ty = s->tile_y ? s->tile_y : 2;
+        int ty = SWEEP_TY;
         // auto: 8-plane interior tiles and 4-plane general tiles, halved
         // while the whole batch has fewer than 8 tiles per CU
         int kb = s->block_z;
@@ -1023,7 +1021,7 @@ static int batch_build(adf_batch* b) {
                 kb /= 2;
             }
         }
-        const int kbg = s->block_z_gen ? s->block_z_gen : std::max(2, kb / 2);
+        const int kbg = s->block_z_gen ? s->block_z_gen : 4;
         for (int par = 0; par < 2; ++par) {
             const int save = s->cur;
             s->cur = par;
@@ -1042,13 +1040,11 @@ static int batch_build(adf_batch* b) {
         if (rc0) return rc0;
         std::vector<uint8_t> open;
         if ((rc0 = open_rows(s, open))) return rc0;
-        std::vector<uint2> gen, in;
-        split_tiles(s, ty, kb, kbg, (uint32_t)m, open, gen, in);
-        const std::tuple<int, int, int> key(fused_cpl(s), ty, s->waves ? s->waves : 1);
-        auto& G = gen_tiles[key];
-        auto& I = int_tiles[key];
-        G.insert(G.end(), gen.begin(), gen.end());
-        I.insert(I.end(), in.begin(), in.end());
+        std::vector<uint2> lists[TILE_LISTS];
+        split_tiles(s, ty, kb, kbg, (uint32_t)m, open, lists);
+        const std::tuple<int, int, int, int> key(fused_cpl(s), ty, s->waves ? s->waves : 1, s->pipeline);
+        auto& G = glists[key];
+        for (int c = 0; c < TILE_LISTS; ++c) G[c].insert(G[c].end(), lists[c].begin(), lists[c].end());
     }
     int rc;
     if ((rc = dmalloc(&b->fs, fs.size() * sizeof(Fused<T>), "batch solves"))) return rc;
@@ -1059,15 +1055,14 @@ static int batch_build(adf_batch* b) {
     HIPCHK(hipMemcpy(b->td, td.data(), td.size() * sizeof(TxDev<T>), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->it0, it0.data(), n * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->cur, cur.data(), n * 4, hipMemcpyHostToDevice));
-    for (auto& kv : gen_tiles) int_tiles[kv.first];  // make sure every key exists
-    for (auto& kv : int_tiles) {
+    for (auto& kv : glists) {
         adf_batch::Group g{};
         g.cpl = std::get<0>(kv.first);
         g.ty = std::get<1>(kv.first);
         g.waves = b->persistent ? 1 : std::get<2>(kv.first);
-        const std::vector<uint2>* lists[2] = {&gen_tiles[kv.first], &kv.second};
-        for (int h = 0; h < 2; ++h) {
-            const std::vector<uint2>& L = *lists[h];
+        g.pf = std::get<3>(kv.first);
+        for (int h = 0; h < TILE_LISTS; ++h) {
+            const std::vector<uint2>& L = kv.second[h];
             g.n[h] = (int)L.size();
             g.grid[h] = b->persistent ? std::min(g.n[h], 16 * b->ctx->num_cu) : (g.n[h] + g.waves - 1) / g.waves;
             g.tiles[h] = nullptr;
@@ -1136,19 +1131,6 @@ static int batch_build(adf_batch* b) {
     return ADF_OK;
 }
 
-template <typename T, bool EXACT, int CPL, int TY>
-static void batch_launch_ty(adf_batch* b, const adf_batch::Group& g, hipStream_t main, hipStream_t side) {
-    const Fused<T>* fs = (const Fused<T>*)b->fs;
-    const int pers = b->persistent ? 1 : 0;
-    const dim3 blk(64 * g.waves);
-    if (g.n[0])
-        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, TY, true>), dim3(g.grid[0]), blk, 0, side, fs, b->cur,
-                           g.tiles[0], g.n[0], g.ctr[0], pers);
-    if (g.n[1])
-        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, TY, false>), dim3(g.grid[1]), blk, 0, main, fs, b->cur,
-                           g.tiles[1], g.n[1], g.ctr[1], pers);
-}
-
 // The launches of one batched step; every per-solve input is read on the
 // device, so the sequence is the same for every step (graph-capturable).
 // The general tiles of all groups run on the side stream, concurrently with
@@ -1158,18 +1140,7 @@ static void batch_step(adf_batch* b, hipStream_t st) {
     hipStream_t side = b->ctx->side;
     (void)hipEventRecord(b->ev_fork, st);
     (void)hipStreamWaitEvent(side, b->ev_fork, 0);
-    for (const auto& g : b->groups) {
-        if (g.cpl == 4) {
-            if (g.ty == 4) batch_launch_ty<T, EXACT, 4, 4>(b, g, st, side);
-            else batch_launch_ty<T, EXACT, 4, 2>(b, g, st, side);
-        } else if (g.cpl == 2) {
-            if (g.ty == 4) batch_launch_ty<T, EXACT, 2, 4>(b, g, st, side);
-            else batch_launch_ty<T, EXACT, 2, 2>(b, g, st, side);
-        } else {
-            if (g.ty == 4) batch_launch_ty<T, EXACT, 1, 4>(b, g, st, side);
-            else batch_launch_ty<T, EXACT, 1, 2>(b, g, st, side);
-        }
-    }
+    launch_fused_batch<T, EXACT>(b, st, side);
     (void)hipEventRecord(b->ev_join, side);
     (void)hipStreamWaitEvent(st, b->ev_join, 0);
     const TxDev<T>* td = (const TxDev<T>*)b->td;

@@ -64,8 +64,8 @@ struct IPlane {
     uint32_t mp_[TY + 1], mvx_[TY + 1], mvy_[TY + 1], mvz_[TY + 1];
 };
 
-template <typename T, bool EXACT, int CPL, int TY, bool PF = false>
-__device__ __forceinline__ void sweep_int_tile(const Fused<T>& F, int tyi, int k0, int k1, bool masked) {
+template <typename T, bool EXACT, int CPL, int TY>
+__device__ __forceinline__ void sweep_int_tile(const Fused<T>& F, int j0, int k0, int k1, bool masked) {
     typedef FA<T, EXACT> M_;
     typedef typename M_::A A;
     constexpr int ES = (int)sizeof(T);
@@ -75,7 +75,6 @@ __device__ __forceinline__ void sweep_int_tile(const Fused<T>& F, int tyi, int k
     const int i0 = lane * CPL;
     const bool lane_ok = i0 < nx;
     const int i0c = lane_ok ? i0 : nx - CPL;
-    const int j0 = tyi * TY;
     const bool cpP = F.cpml_p != 0, cpV = qv > 1;
 
     const uint32_t chb = (uint32_t)F.n3 * (uint32_t)ES;
@@ -214,44 +213,306 @@ __device__ __forceinline__ void sweep_int_tile(const Fused<T>& F, int tyi, int k
             }
     };
 
-    if constexpr (PF) {
-        // software pipeline: the loads of plane k+1 are issued before plane
-        // k is computed (two plane buffers, loop unrolled by two)
-        IPlane<T, CPL, TY> B0, B1;
-        load(B0, k0 - 1);
-        int k = k0 - 1;
-        for (;;) {
-            if (k + 1 < k1) load(B1, k + 1);
-            compute(B0, k);
-            if (++k >= k1) break;
-            if (k + 1 < k1) load(B0, k + 1);
-            compute(B1, k);
-            if (++k >= k1) break;
-        }
-    } else {
-        for (int k = k0 - 1; k < k1; ++k) {
-            IPlane<T, CPL, TY> L;
-            load(L, k);
-            compute(L, k);
-        }
+    for (int k = k0 - 1; k < k1; ++k) {
+        IPlane<T, CPL, TY> L;
+        load(L, k);
+        compute(L, k);
     }
 }
 
-// One solve's tiles (entries as xcd_tile describes), GEN: general tiles
-// (CPML slabs in y or z, boundary planes, clamped rows), else interior
-// tiles; W waves per workgroup take W consecutive tiles.  PF: interior
-// tiles load plane k+1 before computing plane k.
-template <typename T, bool EXACT, int CPL, int TY, bool GEN, bool PF = false>
+// Byte offset that every buffer access treats as out of range: loads at it
+// return zero and stores to it are dropped (the range check covers the
+// vector and instruction offsets, not the scalar one).
+constexpr uint32_t OOR = 0xfffff000u;
+
+// Loaded rows of one plane of a pipelined tile: the interior rows of
+// IPlane plus, for tiles in the y (SY) or z (SZ) CPML slabs, the slab
+// memory rows of both phases.
+template <typename T, int CPL, int TY, bool SY, bool SZ>
+struct SPlane : IPlane<T, CPL, TY> {
+    T mPy[SY ? TY + 1 : 1][CPL], mVy[SY ? TY + 1 : 1][CPL];
+    T mPz[SZ ? TY + 1 : 1][CPL], mVz[SZ ? TY + 1 : 1][CPL];
+    // z slab of the plane (uniform): psi row offsets of row j0 (bytes) and
+    // the coefficients, zero outside the slab
+    int zoP, zoV;
+    bool zinP, zinV;
+    T azP, bzP, azV, bzV;
+};
+
+// The regular-tile routine as a software pipeline.  The loads of plane k+1
+// are issued before plane k is computed and stored, and every memory
+// instruction of the loop body is issued unconditionally (disabled accesses
+// go to a null resource or to the out-of-range offset), so each iteration
+// issues the same sequence and the wait for plane k's data never waits for
+// stores or for the loads of plane k+1.
+//
+// A regular tile keeps away from the mirrored rows/planes (owned rows and
+// planes >= 2), the rigid ones (rows and planes <= n-2) and the grid edge
+// (halo row and plane >= 1, Vy of row j0+TY and Vz of plane k1 inside the
+// grid); the host's tile split guarantees it.  SY / SZ: some row / plane of
+// the tile lies in a y / z CPML slab of either phase; the slab memory update
+// then runs on every row / plane of the tile with zero coefficients (and a
+// zero-reading null resource) outside the slab, which leaves the derivative
+// unchanged, as the reference's slab test does (updates.jl:21-103,192-275,
+// 320-378,380-501).  The x slab is handled the same way per lane.
+template <typename T, bool EXACT, int CPL, int TY, bool SY, bool SZ>
+__device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0, int k1, bool masked) {
+    typedef FA<T, EXACT> M_;
+    typedef typename M_::A A;
+    typedef SPlane<T, CPL, TY, SY, SZ> Plane;
+    constexpr int ES = (int)sizeof(T);
+    const int nx = F.nx, ny = F.ny, nz = F.nz, p = F.p, qv = p + 1;
+    const int sxy = nx * ny;
+    const int lane = threadIdx.x & 63;
+    const int i0 = lane * CPL;
+    const bool lane_ok = i0 < nx;
+    const int i0c = lane_ok ? i0 : nx - CPL;
+    const bool cpP = F.cpml_p != 0, cpV = qv > 1;
+
+    const uint32_t chb = (uint32_t)F.n3 * (uint32_t)ES;
+    const Rsrc rF0 = rsrc_of(F.f0, 4u * chb), rF1 = rsrc_of(F.f1, 4u * chb);
+    const uint32_t psib = (uint32_t)F.psi_n * ES;
+    const Rsrc rS1 = rsrc_of(F.psi1, psib);
+    // null resources (every access out of range) for the disabled streams
+    const Rsrc rSP0 = rsrc_of(F.psi0, cpP ? psib : 0u), rSV0 = rsrc_of(F.psi0, cpV ? psib : 0u);
+    const Rsrc rM = rsrc_of(F.mask, masked ? (uint32_t)(4 * F.nwx * ny * nz) * 4u : 0u);
+    const uint32_t mfill = masked ? 0u : 0xffffffffu;  // all open without a mask
+    const int vl = i0c * ES;
+    const int vs = lane_ok ? i0 * ES : (int)OOR;
+    const int vm = (i0c >> 5) * 4;
+    const int sh = i0c & 31;
+    auto rowb = [&](int j, int k) { return (int)((uint32_t)(nx * j + sxy * k) * (uint32_t)ES); };
+    auto chrow = [&](int c, int rb) { return (int)((uint32_t)c * chb + (uint32_t)rb); };
+    auto maskb = [&](int plane, int j, int k) { return (((plane * nz + k) * ny + j) * F.nwx) * 4; };
+
+    const XSlab<T, CPL> XP = xslab<T, CPL>(F, i0, p, 0, nx - 1, cpP, 0, 2 * p, F.mv_off(0));
+    const XSlab<T, CPL> XV = xslab<T, CPL>(F, i0, qv, 1, nx, cpV, 4 * p, 4 * p + 2 * qv, F.mp_off(0));
+    const int vP = XP.off * ES, vV = XV.off * ES;
+    constexpr uint32_t FULL = (1u << CPL) - 1u;
+    // CPML memory stores of the x-slab cells: one vector store for lanes
+    // whose cells all lie in the slab, one element store per cell for lanes
+    // partly in it; the other lanes' offsets are out of range
+    const bool stP = lane_ok && cpP, stV = lane_ok && cpV;
+    const int sPv = (stP && XP.in == FULL) ? vP : (int)OOR;
+    const int sVv = (stV && XV.in == FULL) ? vV : (int)OOR;
+    int sPq[CPL], sVq[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+        sPq[q] = (stP && XP.in != FULL && ((XP.in >> q) & 1u)) ? vP + q * ES : (int)OOR;
+        sVq[q] = (stV && XV.in != FULL && ((XV.in >> q) & 1u)) ? vV + q * ES : (int)OOR;
+    }
+
+    // y slab of each row (fixed for the tile): psi row offset at plane 0
+    // (bytes; plane k adds k times the plane stride) and the coefficients
+    int yoP[SY ? TY + 1 : 1], yoV[SY ? TY + 1 : 1];
+    bool yinP[SY ? TY + 1 : 1], yinV[SY ? TY + 1 : 1];
+    T ayP[SY ? TY + 1 : 1], byP[SY ? TY + 1 : 1], ayV[SY ? TY + 1 : 1], byV[SY ? TY + 1 : 1];
+    if constexpr (SY) {
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) {
+            const int j = j0 + r - 1;
+            int l = 0;
+            const int sd = cpP ? slab(j, ny, p, l) : 0;
+            yinP[r] = sd != 0;
+            yoP[r] = sd ? (F.mv_off(1) + nx * (l + p * nz * (sd - 1))) * ES : 0;
+            ayP[r] = sd ? F.ca(l + p * (sd - 1)) : (T)0;
+            byP[r] = sd ? F.cb(l + p * (sd - 1)) : (T)0;
+            int lv = 0;
+            const int sv = (cpV && r > 0) ? slab(j, ny, qv, lv) : 0;
+            yinV[r] = sv != 0;
+            yoV[r] = sv ? (F.mp_off(1) + nx * (lv + qv * nz * (sv - 1))) * ES : 0;
+            ayV[r] = sv ? F.cah(lv + qv * (sv - 1)) : (T)0;
+            byV[r] = sv ? F.cbh(lv + qv * (sv - 1)) : (T)0;
+        }
+    }
+    const int ypP = nx * p * ES, ypV = nx * qv * ES;  // y-slab psi plane strides
+
+    T vzc[TY + 1][CPL];
+    T p1prev[TY + 1][CPL];
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) bld<T, CPL>(rF0, vl, chrow(3, rowb(j0 + r - 1, k0 - 1)), vzc[r]);
+
+    auto load = [&](Plane& L, int k) {
+        if constexpr (SZ) {
+            int l = 0;
+            const int sd = cpP ? slab(k, nz, p, l) : 0;
+            L.zinP = sd != 0;
+            L.zoP = sd ? (F.mv_off(2) + nx * (j0 - 1 + ny * (l + p * (sd - 1)))) * ES : 0;
+            L.azP = sd ? F.ca(l + p * (sd - 1)) : (T)0;
+            L.bzP = sd ? F.cb(l + p * (sd - 1)) : (T)0;
+            int lv = 0;
+            const int sv = cpV ? slab(k, nz, qv, lv) : 0;
+            L.zinV = sv != 0;
+            L.zoV = sv ? (F.mp_off(2) + nx * (j0 - 1 + ny * (lv + qv * (sv - 1)))) * ES : 0;
+            L.azV = sv ? F.cah(lv + qv * (sv - 1)) : (T)0;
+            L.bzV = sv ? F.cbh(lv + qv * (sv - 1)) : (T)0;
+        }
+#pragma unroll
+        for (int r = 0; r <= TY + 1; ++r) {
+            const int j = j0 + r - 1;
+            const int rb = rowb(j, k);
+            bld<T, CPL>(rF0, vl, chrow(2, rb), L.vy0[r]);
+            if (r > TY) continue;
+            bld<T, CPL>(rF0, vl, rb, L.p0[r]);
+            bld<T, CPL>(rF0, vl, chrow(1, rb), L.vx0[r]);
+            bld<T, CPL>(rF0, vl, chrow(3, rowb(j, k + 1)), L.vzn[r]);
+            L.mp_[r] = bld_u32(rM, vm, maskb(0, j, k));
+            bld<T, CPL>(rSP0, vP, p * (j + ny * k) * ES, L.mPx[r]);
+            if constexpr (SY)
+                bld<T, CPL>(rsrc_of(F.psi0, yinP[r] ? psib : 0u), vl, yoP[r] + k * ypP, L.mPy[r]);
+            if constexpr (SZ) bld<T, CPL>(rsrc_of(F.psi0, L.zinP ? psib : 0u), vl, L.zoP + r * nx * ES, L.mPz[r]);
+            if (r > 0) {
+                L.mvx_[r] = bld_u32(rM, vm, maskb(1, j, k));
+                L.mvy_[r] = bld_u32(rM, vm, maskb(2, j, k));
+                L.mvz_[r] = bld_u32(rM, vm, maskb(3, j, k));
+                bld<T, CPL>(rSV0, vV, qv * (j + ny * k) * ES, L.mVx[r]);
+                if constexpr (SY)
+                    bld<T, CPL>(rsrc_of(F.psi0, yinV[r] ? psib : 0u), vl, yoV[r] + k * ypV, L.mVy[r]);
+                if constexpr (SZ)
+                    bld<T, CPL>(rsrc_of(F.psi0, L.zinV ? psib : 0u), vl, L.zoV + r * nx * ES, L.mVz[r]);
+            }
+        }
+    };
+
+    auto pressure = [&](Plane& L, T (&p1)[TY + 1][CPL]) {
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) {
+            const T vxl = shfl_down1<T>(L.vx0[r][0]);
+            const uint32_t mp = (L.mp_[r] | mfill) >> sh;
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const T vxn = q + 1 < CPL ? L.vx0[r][q + 1] : vxl;
+                A dvx = M_::diff(vxn, L.vx0[r][q], F.dx, F.rdx);
+                if (cpP) dvx = M_::psi(L.mPx[r][q], XP.b[q], XP.a[q], dvx);
+                A dvy = M_::diff(L.vy0[r + 1][q], L.vy0[r][q], F.dy, F.rdy);
+                if constexpr (SY) dvy = M_::psi(L.mPy[r][q], byP[r], ayP[r], dvy);
+                A dvz = M_::diff(L.vzn[r][q], vzc[r][q], F.dz, F.rdz);
+                if constexpr (SZ) dvz = M_::psi(L.mPz[r][q], L.bzP, L.azP, dvz);
+                const T out = M_::upd(L.p0[r][q], F.fq, F.fqf, (mp >> q) & 1u, (dvx + dvy) + dvz);
+                p1[r][q] = (i0 + q < nx - 1) ? out : L.p0[r][q];
+            }
+        }
+    };
+    auto carry = [&](Plane& L, const T (&p1)[TY + 1][CPL]) {
+#pragma unroll
+        for (int r = 0; r <= TY; ++r)
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                p1prev[r][q] = p1[r][q];
+                vzc[r][q] = L.vzn[r][q];
+            }
+    };
+    // an owned plane: pressure, velocities of the owned rows, all stores
+    auto step = [&](Plane& L, int k) {
+        T p1[TY + 1][CPL];
+        pressure(L, p1);
+#pragma unroll
+        for (int r = 1; r <= TY; ++r) {
+            const int j = j0 + r - 1;
+            const T pml = shfl_up1<T>(p1[r][CPL - 1]);
+            const int rb = rowb(j, k);
+            const uint32_t mvx = (L.mvx_[r] | mfill) >> sh, mvy = (L.mvy_[r] | mfill) >> sh,
+                           mvz = (L.mvz_[r] | mfill) >> sh;
+            T vx1[CPL], vy1[CPL], vz1[CPL];
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const T pm = q > 0 ? p1[r][q - 1] : pml;
+                A dpx = M_::diff(p1[r][q], pm, F.dx, F.rdx);
+                if (cpV) dpx = M_::psi(L.mVx[r][q], XV.b[q], XV.a[q], dpx);
+                const T vx = M_::upd(L.vx0[r][q], F.fv, F.fvf, (mvx >> q) & 1u, dpx);
+                vx1[q] = (i0 + q >= 1) ? vx : L.vx0[r][q];
+                A dpy = M_::diff(p1[r][q], p1[r - 1][q], F.dy, F.rdy);
+                if constexpr (SY) dpy = M_::psi(L.mVy[r][q], byV[r], ayV[r], dpy);
+                vy1[q] = M_::upd(L.vy0[r][q], F.fv, F.fvf, (mvy >> q) & 1u, dpy);
+                A dpz = M_::diff(p1[r][q], p1prev[r][q], F.dz, F.rdz);
+                if constexpr (SZ) dpz = M_::psi(L.mVz[r][q], L.bzV, L.azV, dpz);
+                vz1[q] = M_::upd(vzc[r][q], F.fv, F.fvf, (mvz >> q) & 1u, dpz);
+            }
+            bst<T, CPL>(rF1, vs, rb, p1[r]);
+            bst<T, CPL>(rF1, vs, chrow(1, rb), vx1);
+            bst<T, CPL>(rF1, vs, chrow(2, rb), vy1);
+            bst<T, CPL>(rF1, vs, chrow(3, rb), vz1);
+            const int soP = p * (j + ny * k) * ES, soV = qv * (j + ny * k) * ES;
+            bst<T, CPL>(rS1, sPv, soP, L.mPx[r]);
+            bst<T, CPL>(rS1, sVv, soV, L.mVx[r]);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const T a1[1] = {L.mPx[r][q]}, b1[1] = {L.mVx[r][q]};
+                bst<T, 1>(rS1, sPq[q], soP, a1);
+                bst<T, 1>(rS1, sVq[q], soV, b1);
+            }
+            if constexpr (SY) {
+                bst<T, CPL>(rsrc_of(F.psi1, yinP[r] ? psib : 0u), vs, yoP[r] + k * ypP, L.mPy[r]);
+                bst<T, CPL>(rsrc_of(F.psi1, yinV[r] ? psib : 0u), vs, yoV[r] + k * ypV, L.mVy[r]);
+            }
+            if constexpr (SZ) {
+                bst<T, CPL>(rsrc_of(F.psi1, L.zinP ? psib : 0u), vs, L.zoP + r * nx * ES, L.mPz[r]);
+                bst<T, CPL>(rsrc_of(F.psi1, L.zinV ? psib : 0u), vs, L.zoV + r * nx * ES, L.mVz[r]);
+            }
+        }
+        carry(L, p1);
+    };
+
+    Plane B0, B1;
+    load(B0, k0 - 1);  // halo plane: pressure only
+    load(B1, k0);
+    {
+        T p1[TY + 1][CPL];
+        pressure(B0, p1);
+        carry(B0, p1);
+    }
+    // the scheduling barriers keep each prefetch ahead of the computation
+    // it overlaps (the scheduler would otherwise sink the loads to shorten
+    // register lifetimes)
+    int k = k0;
+    for (;;) {
+        load(B0, min(k + 1, k1 - 1));  // the last prefetch repeats a plane
+        __builtin_amdgcn_sched_barrier(0);
+        step(B1, k);
+        if (++k >= k1) break;
+        load(B1, min(k + 1, k1 - 1));
+        __builtin_amdgcn_sched_barrier(0);
+        step(B0, k);
+        if (++k >= k1) break;
+    }
+}
+
+// Tile kinds: interior tiles (regular, outside every y/z slab), regular
+// tiles in the y and/or z CPML slabs, and general tiles (boundary rows and
+// planes, clamped rows).  Interior tiles form one list, the others the side
+// list, whose entries carry their kind in bits 28-30 of the x word.
+enum { TILE_INT = 0, TILE_SY = 1, TILE_SZ = 2, TILE_SYZ = 3, TILE_GEN = 4 };
+enum { LIST_INT = 0, LIST_SIDE = 1, TILE_LISTS = 2 };
+
+// One tile of a solve from list LIST.  PF: the pipelined routine for the
+// interior and single-slab tiles (tiles in both a y and a z slab, a few
+// per cent of the cells, take fused_tile: their pipelined form would need
+// more registers than the rest of the side list); otherwise the
+// plane-at-a-time routines (fused_tile for all side tiles).
+template <typename T, bool EXACT, int CPL, int TY, int LIST, bool PF>
+__device__ __forceinline__ void sweep_tile(const Fused<T>& F, uint2 e) {
+    const int j0 = (int)(e.x & 0xffffu), k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0xffffu);
+    const bool masked = (e.x >> 31) != 0;
+    if constexpr (LIST == LIST_INT) {
+        if constexpr (PF) sweep_tile_pf<T, EXACT, CPL, TY, false, false>(F, j0, k0, k1, masked);
+        else sweep_int_tile<T, EXACT, CPL, TY>(F, j0, k0, k1, masked);
+    } else {
+        const int kind = (int)((e.x >> 28) & 7u);
+        if (PF && kind == TILE_SY) sweep_tile_pf<T, EXACT, CPL, TY, true, false>(F, j0, k0, k1, masked);
+        else if (PF && kind == TILE_SZ) sweep_tile_pf<T, EXACT, CPL, TY, false, true>(F, j0, k0, k1, masked);
+        else fused_tile<T, EXACT, CPL, TY, true>(F, j0, k0, k1, masked);
+    }
+}
+
+// One solve's tiles of one list (entries as xcd_tile describes); W waves
+// per workgroup take W consecutive tiles.
+template <typename T, bool EXACT, int CPL, int TY, int LIST, bool PF>
 __global__ __launch_bounds__(256) void k_sweep(Fused<T> F, const uint2* __restrict__ tiles, int ntiles) {
     const int W = (int)(blockDim.x >> 6);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x) * W + w;
     if (t >= ntiles) return;
-    const uint2 e = tiles[t];
-    const int tyi = (int)(e.x & 0xffffu), k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0xffffu);
-    const bool masked = (e.x >> 31) != 0;
-    if constexpr (GEN) fused_tile<T, EXACT, CPL, TY, true>(F, tyi, k0, k1, masked);
-    else sweep_int_tile<T, EXACT, CPL, TY, PF>(F, tyi, k0, k1, masked);
+    sweep_tile<T, EXACT, CPL, TY, LIST, PF>(F, tiles[t]);
 }
 
 }  // namespace adf

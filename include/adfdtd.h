@@ -179,7 +179,8 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
 /* Kernel path and tiling (tuning knobs; results do not depend on them):
  *   ADF_OPT_PATH     0 auto (fused when nx <= 256), 1 two-pass P/V kernels,
  *                    2 fused z-marching kernel
- *   ADF_OPT_TILE_Y   rows per wave of the fused kernel: 0 auto, 2, 4
+ *   ADF_OPT_TILE_Y   rows per wave of the fused kernel: 0 auto, 2 (the only
+ *                    shape built)
  *   ADF_OPT_BLOCK_Z  planes per workgroup of the fused kernel: 0 auto, >= 1 */
 #define ADF_OPT_PATH 1
 #define ADF_OPT_TILE_Y 2
@@ -192,12 +193,14 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
 /*   ADF_OPT_WAVES    waves per workgroup of the fused kernel: 0 auto, 1, 2, 4
  *                    (a workgroup takes that many y-neighbouring tiles) */
 #define ADF_OPT_WAVES 5
-/*   ADF_OPT_BLOCK_Z_GEN  planes per tile of the fused kernel's general tiles
- *                        (CPML slabs in y/z, boundary planes): 0 auto, >= 1 */
+/*   ADF_OPT_BLOCK_Z_GEN  planes per tile of the fused kernel's side tiles
+ *                        (CPML slabs in y/z, boundary rows and planes):
+ *                        0 auto, >= 1 */
 #define ADF_OPT_BLOCK_Z_GEN 6
-/*   ADF_OPT_PIPELINE     1: interior tiles of the fused kernel load the next
- *                        plane while the current one is computed (two plane
- *                        buffers in registers); 0: one plane at a time */
+/*   ADF_OPT_PIPELINE     1 (default): the fused kernel's regular tiles load
+ *                        the next plane while the current one is computed
+ *                        (two plane buffers in registers); 0: one plane at
+ *                        a time */
 #define ADF_OPT_PIPELINE 7
 /*   ADF_OPT_CONCURRENT   1 (default): general and interior tiles of the fused
  *                        kernel run as two launches on two streams at the
@@ -227,8 +230,8 @@ int adf_solver_next_step(adf_solver* s, int64_t* it);
  * domain, CPML, mask, transceivers and step counter; a batched step of a
  * member is the same forward_onestep! (src/kernels/forward.jl:13-139) that
  * adf_solver_run takes, value for value, whatever else is in the batch.
- * One step of the batch runs the field sweep of all members as one
- * persistent tile-queue launch per row-width class, the transceiver levels
+ * One step of the batch runs the field sweep of all members as two launches
+ * (interior tiles, side tiles) per row-width class, the transceiver levels
  * of all members as one launch per level, then the velocity fixups and
  * mirrors; it is captured as a HIP graph and replayed.
  * Members: same context and dtype, nx <= 256 and nx divisible by its cells
@@ -243,8 +246,8 @@ int adf_batch_size(adf_batch* b, int64_t* n);
  * (adf_solver_next_step); fails if a member would pass its nt.
  * Synchronous on return. */
 int adf_batch_run(adf_batch* b, int64_t nsteps);
-/*   ADF_BATCH_PERSISTENT   1 (default): a fixed grid of waves drains an
- *                          atomic tile queue; 0: one workgroup per tile
+/*   ADF_BATCH_PERSISTENT   1: a fixed grid of waves drains an atomic tile
+ *                          queue; 0 (default): one workgroup per tile
  *   ADF_BATCH_GRAPH_STEPS  steps per captured HIP graph (default 16; 0 runs
  *                          the launches directly) */
 #define ADF_BATCH_PERSISTENT 1

@@ -135,7 +135,7 @@ def test_tilings_do_not_change_results():
         s.run(1, s.dom.nt)
     b = A.Batch()
     mem = [make_solver(c, np.float32, tile_y=ty, block_z=kb, waves=w)
-           for c, (ty, kb, w) in zip(cs, [(4, 3, 2), (4, 0, 1), (2, 7, 4)])]
+           for c, (ty, kb, w) in zip(cs, [(0, 3, 2), (0, 0, 1), (0, 7, 4)])]
     for s in mem:
         b.add(s)
     b.run(cs[0][0].nt)

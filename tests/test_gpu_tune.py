@@ -28,7 +28,8 @@ def test_tune_solver_cached_and_result_invariant(tmp_path):
     ref.run()
     s, g = _solver(case)
     shape = autotune.tune_solver(s, steps=6, path=path)
-    assert {k: shape[k] for k in ("tile_y", "block_z", "block_z_gen", "waves")} in autotune.SOLVER_CANDIDATES
+    keys = ("tile_y", "block_z", "block_z_gen", "waves", "pipeline", "concurrent")
+    assert {k: shape[k] for k in keys} in autotune.SOLVER_CANDIDATES
     data = json.load(open(path))
     assert len(data) == 1 and next(iter(data)).startswith("solver:")
     assert autotune.tune_solver(s, path=path) == shape  # cache hit
@@ -58,7 +59,7 @@ def test_tune_batch_cached_and_result_invariant(tmp_path):
     b = A.Batch()
     cands = autotune.BATCH_CANDIDATES[::7]
     shape = autotune.tune_batch(mem, steps=6, candidates=cands, path=path, batch=b)
-    assert {k: shape[k] for k in ("tile_y", "block_z", "block_z_gen", "waves", "persistent")} in cands
+    assert {k: shape[k] for k in ("tile_y", "block_z", "block_z_gen", "waves", "persistent", "pipeline")} in cands
     assert next(iter(json.load(open(path)))).startswith("batch:")
     for s in mem:
         b.add(s)
