@@ -151,7 +151,6 @@ struct adf_batch {
     hipGraphExec_t gexec = nullptr;
     int gexec_steps = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double last_step_ms = 0.0;
 };
 

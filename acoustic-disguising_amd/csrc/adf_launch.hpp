@@ -20,9 +20,11 @@ constexpr int SWEEP_TY_SIDE = 1;
 template <typename T, bool EXACT>
 void launch_fused(adf_solver* s, const Fused<T>& F, int cpl, hipStream_t st);
 
-// One sweep of every member of batch b: interior tiles on `main`, the other
-// tiles on `side`.
+// One sweep of every member of batch b on stream st: the side tiles, then
+// the interior tiles.  One stream: a batched step is captured in a HIP
+// graph, where the two-stream fork/join of the single-solve sweep gave
+// intermittently different results on replay.
 template <typename T, bool EXACT>
-void launch_fused_batch(adf_batch* b, hipStream_t main, hipStream_t side);
+void launch_fused_batch(adf_batch* b, hipStream_t st);
 
 }  // namespace adf

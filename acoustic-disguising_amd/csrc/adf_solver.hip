@@ -685,6 +685,11 @@ static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, uint32_t m
     blocks(zb, zi1, rb, kbg);
     blocks(zb, rb, nz, kbg);
     blocks(zs, 0, nz, kbg);
+    // rows handled by the general routine throughout take one plane per
+    // tile: that routine waits out a memory round trip per plane, so its
+    // chains are kept short
+    std::vector<std::pair<int, int>> z1;
+    blocks(z1, 0, nz, 1);
     // (k0, t, entry) in (z-block, y-tile) order, so neighbouring entries
     // share halo rows
     // Rows are classified in tiles of ty rows; side tiles are then cut into
@@ -694,7 +699,7 @@ static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, uint32_t m
     for (int t = 0; t < tiles_y; ++t) {
         const bool yg = rgen(t), ys = !yg && rslab(t);
         const int j0 = t * ty, j1 = std::min(j0 + ty, ny);
-        for (auto& z : (yg || ys) ? zs : zb) {
+        for (auto& z : yg ? z1 : (ys ? zs : zb)) {
             uint32_t kind = TILE_GEN;
             if (!yg && z.first >= 2 && z.second <= nz - 1) {
                 const bool zl = zslab(z.first, z.second);
@@ -964,8 +969,7 @@ void batch_free(adf_batch* b) {
     batch_drop_tables(b);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
-    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
-    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+
 }
 
 int batch_add(adf_batch* b, adf_solver* s) {
@@ -1131,18 +1135,12 @@ static int batch_build(adf_batch* b) {
     return ADF_OK;
 }
 
-// The launches of one batched step; every per-solve input is read on the
-// device, so the sequence is the same for every step (graph-capturable).
-// The general tiles of all groups run on the side stream, concurrently with
-// the interior tiles on the main stream.
+// The launches of one batched step, all on stream st; every per-solve input
+// is read on the device, so the sequence is the same for every step
+// (graph-capturable).
 template <typename T, bool EXACT>
 static void batch_step(adf_batch* b, hipStream_t st) {
-    hipStream_t side = b->ctx->side;
-    (void)hipEventRecord(b->ev_fork, st);
-    (void)hipStreamWaitEvent(side, b->ev_fork, 0);
-    launch_fused_batch<T, EXACT>(b, st, side);
-    (void)hipEventRecord(b->ev_join, side);
-    (void)hipStreamWaitEvent(st, b->ev_join, 0);
+    launch_fused_batch<T, EXACT>(b, st);
     const TxDev<T>* td = (const TxDev<T>*)b->td;
     for (size_t L = 0; L + 1 < b->txp_lev.size(); ++L) {
         const int n = b->txp_lev[L + 1] - b->txp_lev[L];
@@ -1183,8 +1181,6 @@ static int batch_run_impl(adf_batch* b, int64_t nsteps) {
     if (!b->ev0) {
         HIPCHK(hipEventCreate(&b->ev0));
         HIPCHK(hipEventCreate(&b->ev1));
-        HIPCHK(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
     }
     int64_t left = nsteps;
     const int G = b->graph_steps;

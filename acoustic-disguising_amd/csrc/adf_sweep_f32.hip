@@ -3,5 +3,5 @@
 
 namespace adf {
 template void launch_fused<float, false>(adf_solver*, const Fused<float>&, int, hipStream_t);
-template void launch_fused_batch<float, false>(adf_batch*, hipStream_t, hipStream_t);
+template void launch_fused_batch<float, false>(adf_batch*, hipStream_t);
 }  // namespace adf

@@ -3,5 +3,5 @@
 
 namespace adf {
 template void launch_fused<double, true>(adf_solver*, const Fused<double>&, int, hipStream_t);
-template void launch_fused_batch<double, true>(adf_batch*, hipStream_t, hipStream_t);
+template void launch_fused_batch<double, true>(adf_batch*, hipStream_t);
 }  // namespace adf

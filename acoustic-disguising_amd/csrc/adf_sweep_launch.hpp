@@ -63,17 +63,17 @@ static void batch_launch_kind(adf_batch* b, const adf_batch::Group& g, hipStream
 }
 
 template <typename T, bool EXACT, int CPL, int TY>
-static void batch_launch_ty(adf_batch* b, const adf_batch::Group& g, hipStream_t main, hipStream_t side) {
-    batch_launch_kind<T, EXACT, CPL, SWEEP_TY_SIDE, LIST_SIDE>(b, g, side);
-    batch_launch_kind<T, EXACT, CPL, TY, LIST_INT>(b, g, main);
+static void batch_launch_ty(adf_batch* b, const adf_batch::Group& g, hipStream_t st) {
+    batch_launch_kind<T, EXACT, CPL, SWEEP_TY_SIDE, LIST_SIDE>(b, g, st);
+    batch_launch_kind<T, EXACT, CPL, TY, LIST_INT>(b, g, st);
 }
 
 template <typename T, bool EXACT>
-void launch_fused_batch(adf_batch* b, hipStream_t st, hipStream_t side) {
+void launch_fused_batch(adf_batch* b, hipStream_t st) {
     for (const auto& g : b->groups) {
-        if (g.cpl == 4) batch_launch_ty<T, EXACT, 4, SWEEP_TY>(b, g, st, side);
-        else if (g.cpl == 2) batch_launch_ty<T, EXACT, 2, SWEEP_TY>(b, g, st, side);
-        else batch_launch_ty<T, EXACT, 1, SWEEP_TY>(b, g, st, side);
+        if (g.cpl == 4) batch_launch_ty<T, EXACT, 4, SWEEP_TY>(b, g, st);
+        else if (g.cpl == 2) batch_launch_ty<T, EXACT, 2, SWEEP_TY>(b, g, st);
+        else batch_launch_ty<T, EXACT, 1, SWEEP_TY>(b, g, st);
     }
 }
 
