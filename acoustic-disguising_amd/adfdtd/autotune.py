@@ -1,7 +1,7 @@
 """Tile-shape autotuner for the fused sweep, with a JSON cache.
 
 The sweep's tiling (planes per interior tile, planes per side tile, waves
-per workgroup, the pipelined routines, concurrent launches and, for
+per workgroup, the pipelined routines, the launch form and, for
 batches, the persistent tile queue) changes only speed, never results.  The tuner times the candidates on
 scratch copies of the solves (same grids, CPML, masks and dtype, zero
 fields, no transceivers), so the caller's state is untouched, and stores
@@ -24,10 +24,11 @@ from .solver import Solver
 _lock = threading.Lock()
 
 # block_z: planes per interior tile; block_z_gen: planes per side tile (CPML
-# slabs and boundary); pipeline: the prefetching tile routines; concurrent:
-# interior and side tiles as two simultaneous launches (single solves)
-SOLVER_CANDIDATES = [dict(tile_y=0, block_z=kb, block_z_gen=kbg, waves=1, pipeline=pf, concurrent=cc)
-                     for kb, kbg, pf, cc in itertools.product((8, 16, 32), (2, 4), (1, 0), (1, 0))]
+# slabs and boundary); pipeline: the prefetching tile routines; launch: one
+# launch over all tiles, or side and interior tiles as two launches on two
+# streams at the same time (single solves; ADF_SWEEP_*)
+SOLVER_CANDIDATES = [dict(tile_y=0, block_z=kb, block_z_gen=kbg, waves=1, pipeline=pf, launch=ln)
+                     for kb, kbg, pf, ln in itertools.product((4, 8, 16, 32), (2, 4), (1, 0), (0, 1))]
 BATCH_CANDIDATES = [dict(tile_y=0, block_z=kb, block_z_gen=kbg, waves=1, persistent=pers, pipeline=pf)
                     for kb, kbg, pers, pf in itertools.product((4, 8, 16), (2, 4), (0, 1), (1, 0))]
 
@@ -72,8 +73,7 @@ def _scratch(s: Solver) -> Solver:
 
 def _apply(s: Solver, shape: dict) -> None:
     s.set_path("auto", shape["tile_y"], shape["block_z"], shape["waves"], shape["block_z_gen"],
-               pipeline=shape.get("pipeline", 1))
-    s.set_option(L.ADF_OPT_CONCURRENT, shape.get("concurrent", 1))
+               pipeline=shape.get("pipeline", 1), launch=shape.get("launch", L.ADF_SWEEP_ONE_LAUNCH))
 
 
 def tune_solver(s: Solver, steps: int = 20, candidates: Optional[Sequence[dict]] = None,

@@ -30,7 +30,7 @@ struct TxDev {
 // takes tiles from chunk x's queue (ctr[x]) and, when it is empty, from the
 // other chunks' queues in turn; the last wave to finish resets the queue
 // heads for the next launch.
-template <typename T, bool EXACT, int CPL, int TY, int LIST, bool PF>
+template <typename T, bool EXACT, int CPL, bool PF>
 __global__ __launch_bounds__(256) void k_sweep_batch(const Fused<T>* __restrict__ fs, const int* __restrict__ cur,
                                                      const uint2* __restrict__ tiles, int ntiles, unsigned* ctr,
                                                      int persistent) {
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void k_sweep_batch(const Fused<T>* __restrict_
         const uint2 e = tiles[t];
         const int s = (int)((e.x >> 16) & 0xfffu);
         const Fused<T> F = fs[2 * s + cur[s]];
-        sweep_tile<T, EXACT, CPL, TY, LIST, PF>(F, e);
+        sweep_tile<T, EXACT, CPL, PF>(F, e);
     };
     if (!persistent) {
         const int W = (int)(blockDim.x >> 6);

@@ -27,6 +27,12 @@
 
 namespace adf {
 
+// Rows per wave of the fused sweep's tiles: interior tiles (SWEEP_TY, also
+// the row granularity of the tile split) and side tiles (CPML-slab and
+// boundary tiles, whose routines need more registers per row).
+constexpr int SWEEP_TY = 2;
+constexpr int SWEEP_TY_SIDE = 1;
+
 // Kernel arguments of one solve, kept small so the fused kernel does not
 // run out of scalar registers: one base pointer per buffer, offsets derived
 // on the device.  f0/f1: old/new field (P,Vx,Vy,Vz at c*n3); psi0/psi1:

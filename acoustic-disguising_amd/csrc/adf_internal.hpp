@@ -23,8 +23,9 @@ int fail(int code, const std::string& msg);
 struct adf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // second stream of the same (default) priority: the general tiles of a
-    // sweep run on it, concurrently with the interior tiles on `stream`
+    // second stream of the same (default) priority: with
+    // ADF_SWEEP_TWO_STREAMS the side tiles of a sweep run on it, at the same
+    // time as the interior tiles on `stream`
     hipStream_t side = nullptr;
     int num_cu = 256;
 };
@@ -79,11 +80,12 @@ struct adf_solver {
     uint32_t* fix_cells = nullptr;
     int n_fix = 0;
 
-    // fused-sweep tile lists for tiling (tl_ty, tl_kb, tl_kbg): interior
-    // tiles and the side list (slab and general tiles; LIST_* order)
-    int tl_ty = 0, tl_kb = 0, tl_kbg = 0;
-    uint2* tiles[2] = {nullptr, nullptr};
-    int ntl[2] = {0, 0};
+    // fused-sweep tile lists for tiling (tl_ty, tl_kb, tl_kbg, tl_w):
+    // interior tiles, side tiles (CPML-slab and boundary) and all tiles
+    // (LIST_* order)
+    int tl_ty = 0, tl_kb = 0, tl_kbg = 0, tl_w = 0;
+    uint2* tiles[3] = {nullptr, nullptr, nullptr};
+    int ntl[3] = {0, 0, 0};
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     // kernel path and tiling (adf_solver_set_option)
@@ -93,7 +95,7 @@ struct adf_solver {
     int waves = 0;     // waves per workgroup of the fused kernel (0 auto)
     int block_z_gen = 0;  // planes per general tile (0 auto)
     int pipeline = 1;     // pipelined (prefetching) tile routines
-    int concurrent = 1;   // general and interior launches on two streams
+    int launch = 0;       // ADF_SWEEP_* (one launch, two streams, two serial)
 
     // GF extrapolation map
     int gf_kind = 0;
@@ -134,9 +136,10 @@ struct adf_batch {
     // one group per (cells per lane, rows per wave, waves per workgroup);
     struct Group {
         int cpl, ty, waves, pf;  // pf: pipelined tile routines
-        int n[2], grid[2];      // per tile list (LIST_* order)
-        uint2* tiles[2];
-        unsigned* ctr[2];       // 8 per-XCD queue heads + a finished-wave count
+        int merged;             // one launch over LIST_ALL, else LIST_SIDE then LIST_INT
+        int n[3], grid[3];      // per tile list (LIST_* order)
+        uint2* tiles[3];
+        unsigned* ctr[3];       // 8 per-XCD queue heads + a finished-wave count
     };
     std::vector<Group> groups;
     uint32_t* txp = nullptr;

@@ -8,15 +8,9 @@
 
 namespace adf {
 
-// Rows per wave of the fused sweep's tiles: interior tiles (SWEEP_TY, the
-// row granularity of the tile split) and side tiles (slab and general
-// tiles, whose routines need more registers per row).
-constexpr int SWEEP_TY = 2;
-constexpr int SWEEP_TY_SIDE = 1;
-
 // One sweep of solve s (old buffers -> new buffers as F describes), cpl
-// cells per lane, on stream st (general and slab tiles on the context's
-// side stream when s->concurrent).
+// cells per lane, on stream st (side tiles on the context's side stream
+// when s->launch is ADF_SWEEP_TWO_STREAMS).
 template <typename T, bool EXACT>
 void launch_fused(adf_solver* s, const Fused<T>& F, int cpl, hipStream_t st);
 

@@ -442,27 +442,35 @@ int solver_set_option(adf_solver* s, int option, int64_t value) {
         if (value < 0 || value > 2) return fail(ADF_EINVAL, "ADF_OPT_PATH must be 0, 1 or 2");
         if (value == 2 && !fused_supported(s)) return fail(ADF_EINVAL, "fused path needs 2 <= nx <= 256");
         s->path = (int)value;
+        touch(s);
     } else if (option == ADF_OPT_TILE_Y) {
         if (value != 0 && value != SWEEP_TY)
             return fail(ADF_EINVAL, "ADF_OPT_TILE_Y must be 0 or " + std::to_string(SWEEP_TY));
         s->tile_y = (int)value;
+        touch(s);
     } else if (option == ADF_OPT_TIME_SWEEP) {
         s->time_sweep = value != 0;
     } else if (option == ADF_OPT_BLOCK_Z) {
         if (value < 0 || value > (1 << 20)) return fail(ADF_EINVAL, "ADF_OPT_BLOCK_Z out of range");
         s->block_z = (int)value;
+        touch(s);
     } else if (option == ADF_OPT_BLOCK_Z_GEN) {
         if (value < 0 || value > 65535) return fail(ADF_EINVAL, "ADF_OPT_BLOCK_Z_GEN out of range");
         s->block_z_gen = (int)value;
+        touch(s);
     } else if (option == ADF_OPT_WAVES) {
         if (value != 0 && value != 1 && value != 2 && value != 4) return fail(ADF_EINVAL, "ADF_OPT_WAVES must be 0, 1, 2 or 4");
         s->waves = (int)value;
+        touch(s);
     } else if (option == ADF_OPT_PIPELINE) {
         if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_PIPELINE must be 0 or 1");
         s->pipeline = (int)value;
-    } else if (option == ADF_OPT_CONCURRENT) {
-        if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_CONCURRENT must be 0 or 1");
-        s->concurrent = (int)value;
+        touch(s);
+    } else if (option == ADF_OPT_SWEEP_LAUNCH) {
+        if (value < ADF_SWEEP_ONE_LAUNCH || value > ADF_SWEEP_TWO_LAUNCHES)
+            return fail(ADF_EINVAL, "ADF_OPT_SWEEP_LAUNCH must be 0, 1 or 2");
+        s->launch = (int)value;
+        touch(s);
     } else {
         return fail(ADF_EINVAL, "unknown option");
     }
@@ -540,15 +548,16 @@ static int fused_cpl(const adf_solver* s) {
 
 bool fused_supported(const adf_solver* s) { return fused_cpl(s) > 0; }
 
-// Tile shape: TY rows per wave, KB planes per interior tile and KBG planes
-// per general tile, chosen so the launch has several waves per CU while
-// halo work stays small; general tiles (slower per plane) get fewer planes.
+// Tile shape: TY rows per interior tile, KB planes per interior tile (halved
+// from 64 until the launch has 16 tiles per CU: short dependency chains
+// and even work per CU matter more than the halo plane each tile reloads)
+// and KBG planes per side tile.
 void pick_tiles(const adf_solver* s, int* ty, int* kb, int* kbg) {
     int t = SWEEP_TY;
     int k = s->block_z;
     if (!k) {
         const int64_t ny = s->dom.ny, nz = s->dom.nz;
-        const int64_t want = 8LL * s->ctx->num_cu;
+        const int64_t want = 16LL * s->ctx->num_cu;
         const int64_t tiles_y = (ny + t - 1) / t;
         k = 64;
         while (k > 8 && tiles_y * ((nz + k - 1) / k) < want) k /= 2;
@@ -630,8 +639,29 @@ static int open_rows(const adf_solver* s, std::vector<uint8_t>& open) {
     return ADF_OK;
 }
 
-static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, uint32_t m, const std::vector<uint8_t>& open,
-                        std::vector<uint2> (&out)[TILE_LISTS]) {
+// Interior and side lists in (z-block, row) order, then both merged into
+// LIST_ALL: a launch of W waves per workgroup gives XCD x the x-th of 8
+// contiguous chunks of the list (xcd_tile), so chunk x holds its share of
+// the side tiles first (their short chains start early) and then its share
+// of the interior tiles, each share taken in list order.
+static void merge_lists(const std::vector<uint2>& side, const std::vector<uint2>& in, int W, std::vector<uint2>& out) {
+    const int64_t S = (int64_t)side.size(), n = S + (int64_t)in.size();
+    const int64_t nb = (n + W - 1) / W;
+    auto start = [&](int x) {  // first tile of chunk x (xcd_chunk_start over blocks)
+        const int64_t b = x * (nb >> 3) + std::min<int64_t>(x, nb & 7);
+        return std::min<int64_t>(b * W, n);
+    };
+    out.clear();
+    for (int x = 0; x < 8; ++x) {
+        const int64_t a = start(x), b = start(x + 1);
+        const int64_t sa = n ? S * a / n : 0, sb = n ? S * b / n : 0;
+        out.insert(out.end(), side.begin() + sa, side.begin() + sb);
+        out.insert(out.end(), in.begin() + (a - sa), in.begin() + (b - sb));
+    }
+}
+
+static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, int W, uint32_t m,
+                        const std::vector<uint8_t>& open, std::vector<uint2> (&out)[TILE_LISTS]) {
     const int ny = (int)s->dom.ny, nz = (int)s->dom.nz, p = (int)s->p, qv = p + 1;
     const bool cpP = s->p > 0 && (s->has_mask || s->p > 1), cpV = qv > 1;
     const int tiles_y = (ny + ty - 1) / ty;
@@ -716,11 +746,12 @@ static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, uint32_t m
     auto order = [](const std::tuple<int, int, uint2>& a, const std::tuple<int, int, uint2>& b) {
         return std::get<0>(a) != std::get<0>(b) ? std::get<0>(a) < std::get<0>(b) : std::get<1>(a) < std::get<1>(b);
     };
-    for (int c = 0; c < TILE_LISTS; ++c) {
+    for (int c : {LIST_INT, LIST_SIDE}) {
         std::sort(lists[c].begin(), lists[c].end(), order);
         out[c].clear();
         for (auto& e : lists[c]) out[c].push_back(std::get<2>(e));
     }
+    merge_lists(out[LIST_SIDE], out[LIST_INT], W, out[LIST_ALL]);
 }
 
 static int upload_tiles(const std::vector<uint2>& v, uint2** dst, int* n) {
@@ -742,13 +773,14 @@ static int check_tiling(const adf_solver* s, int ty) {
 }
 
 static int prepare_tiles(adf_solver* s, int ty, int kb, int kbg) {
-    if (s->tl_ty == ty && s->tl_kb == kb && s->tl_kbg == kbg) return ADF_OK;
+    const int W = s->waves ? s->waves : 1;
+    if (s->tl_ty == ty && s->tl_kb == kb && s->tl_kbg == kbg && s->tl_w == W) return ADF_OK;
     int rc = check_tiling(s, ty);
     if (rc) return rc;
     std::vector<uint8_t> open;
     if ((rc = open_rows(s, open))) return rc;
     std::vector<uint2> lists[TILE_LISTS];
-    split_tiles(s, ty, kb, kbg, 0, open, lists);
+    split_tiles(s, ty, kb, kbg, W, 0, open, lists);
     for (int c = 0; c < TILE_LISTS; ++c)
         if ((rc = upload_tiles(lists[c], &s->tiles[c], &s->ntl[c]))) return rc;
     if (!s->ev_fork) {
@@ -758,6 +790,7 @@ static int prepare_tiles(adf_solver* s, int ty, int kb, int kbg) {
     s->tl_ty = ty;
     s->tl_kb = kb;
     s->tl_kbg = kbg;
+    s->tl_w = W;
     return ADF_OK;
 }
 
@@ -1005,7 +1038,7 @@ static int batch_build(adf_batch* b) {
     std::vector<Fused<T>> fs(2 * n);
     std::vector<TxDev<T>> td(n);
     std::vector<int> it0(n), cur(n);
-    std::map<std::tuple<int, int, int, int>, std::array<std::vector<uint2>, TILE_LISTS>> glists;
+    std::map<std::tuple<int, int, int, int, int>, std::array<std::vector<uint2>, TILE_LISTS>> glists;
     // total work decides the z-block: enough tiles for every CU
     int64_t cells = 0;
     for (adf_solver* s : b->members) cells += s->n3;
@@ -1045,10 +1078,11 @@ static int batch_build(adf_batch* b) {
         std::vector<uint8_t> open;
         if ((rc0 = open_rows(s, open))) return rc0;
         std::vector<uint2> lists[TILE_LISTS];
-        split_tiles(s, ty, kb, kbg, (uint32_t)m, open, lists);
-        const std::tuple<int, int, int, int> key(fused_cpl(s), ty, s->waves ? s->waves : 1, s->pipeline);
+        split_tiles(s, ty, kb, kbg, 1, (uint32_t)m, open, lists);
+        const std::tuple<int, int, int, int, int> key(fused_cpl(s), ty, s->waves ? s->waves : 1, s->pipeline,
+                                                      s->launch == ADF_SWEEP_ONE_LAUNCH ? 1 : 0);
         auto& G = glists[key];
-        for (int c = 0; c < TILE_LISTS; ++c) G[c].insert(G[c].end(), lists[c].begin(), lists[c].end());
+        for (int c : {LIST_INT, LIST_SIDE}) G[c].insert(G[c].end(), lists[c].begin(), lists[c].end());
     }
     int rc;
     if ((rc = dmalloc(&b->fs, fs.size() * sizeof(Fused<T>), "batch solves"))) return rc;
@@ -1065,6 +1099,8 @@ static int batch_build(adf_batch* b) {
         g.ty = std::get<1>(kv.first);
         g.waves = b->persistent ? 1 : std::get<2>(kv.first);
         g.pf = std::get<3>(kv.first);
+        g.merged = std::get<4>(kv.first);
+        if (g.merged) merge_lists(kv.second[LIST_SIDE], kv.second[LIST_INT], g.waves, kv.second[LIST_ALL]);
         for (int h = 0; h < TILE_LISTS; ++h) {
             const std::vector<uint2>& L = kv.second[h];
             g.n[h] = (int)L.size();

@@ -477,42 +477,47 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
     }
 }
 
-// Tile kinds: interior tiles (regular, outside every y/z slab), regular
-// tiles in the y and/or z CPML slabs, and general tiles (boundary rows and
-// planes, clamped rows).  Interior tiles form one list, the others the side
-// list, whose entries carry their kind in bits 28-30 of the x word.
+// Tile kinds: interior tiles (regular, outside every y/z slab; SWEEP_TY
+// rows), regular tiles in the y and/or z CPML slabs, and general tiles
+// (boundary rows and planes, clamped rows); the last three are side tiles
+// (SWEEP_TY_SIDE rows).  Entries carry their kind in bits 28-30 of the x
+// word.  Lists: interior tiles, side tiles, or all tiles (side tiles first
+// within each XCD's chunk).
 enum { TILE_INT = 0, TILE_SY = 1, TILE_SZ = 2, TILE_SYZ = 3, TILE_GEN = 4 };
-enum { LIST_INT = 0, LIST_SIDE = 1, TILE_LISTS = 2 };
+enum { LIST_INT = 0, LIST_SIDE = 1, LIST_ALL = 2, TILE_LISTS = 3 };
 
-// One tile of a solve from list LIST.  PF: the pipelined routine for the
-// interior and single-slab tiles (tiles in both a y and a z slab, a few
-// per cent of the cells, take fused_tile: their pipelined form would need
-// more registers than the rest of the side list); otherwise the
-// plane-at-a-time routines (fused_tile for all side tiles).
-template <typename T, bool EXACT, int CPL, int TY, int LIST, bool PF>
+// One tile of a solve.  PF: the pipelined routine for interior and
+// single-slab tiles (tiles in both a y and a z slab, a few per cent of the
+// cells, take fused_tile: their pipelined form needs more registers than
+// the rest); otherwise the plane-at-a-time routines (fused_tile for all
+// side tiles).
+template <typename T, bool EXACT, int CPL, bool PF>
 __device__ __forceinline__ void sweep_tile(const Fused<T>& F, uint2 e) {
     const int j0 = (int)(e.x & 0xffffu), k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0xffffu);
     const bool masked = (e.x >> 31) != 0;
-    if constexpr (LIST == LIST_INT) {
-        if constexpr (PF) sweep_tile_pf<T, EXACT, CPL, TY, false, false>(F, j0, k0, k1, masked);
-        else sweep_int_tile<T, EXACT, CPL, TY>(F, j0, k0, k1, masked);
+    const int kind = (int)((e.x >> 28) & 7u);
+    constexpr int TI = SWEEP_TY, TS = SWEEP_TY_SIDE;
+    if (kind == TILE_INT) {
+        if constexpr (PF) sweep_tile_pf<T, EXACT, CPL, TI, false, false>(F, j0, k0, k1, masked);
+        else sweep_int_tile<T, EXACT, CPL, TI>(F, j0, k0, k1, masked);
+    } else if (PF && kind == TILE_SY) {
+        sweep_tile_pf<T, EXACT, CPL, TS, true, false>(F, j0, k0, k1, masked);
+    } else if (PF && kind == TILE_SZ) {
+        sweep_tile_pf<T, EXACT, CPL, TS, false, true>(F, j0, k0, k1, masked);
     } else {
-        const int kind = (int)((e.x >> 28) & 7u);
-        if (PF && kind == TILE_SY) sweep_tile_pf<T, EXACT, CPL, TY, true, false>(F, j0, k0, k1, masked);
-        else if (PF && kind == TILE_SZ) sweep_tile_pf<T, EXACT, CPL, TY, false, true>(F, j0, k0, k1, masked);
-        else fused_tile<T, EXACT, CPL, TY, true>(F, j0, k0, k1, masked);
+        fused_tile<T, EXACT, CPL, TS, true>(F, j0, k0, k1, masked);
     }
 }
 
 // One solve's tiles of one list (entries as xcd_tile describes); W waves
 // per workgroup take W consecutive tiles.
-template <typename T, bool EXACT, int CPL, int TY, int LIST, bool PF>
+template <typename T, bool EXACT, int CPL, bool PF>
 __global__ __launch_bounds__(256) void k_sweep(Fused<T> F, const uint2* __restrict__ tiles, int ntiles) {
     const int W = (int)(blockDim.x >> 6);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x) * W + w;
     if (t >= ntiles) return;
-    sweep_tile<T, EXACT, CPL, TY, LIST, PF>(F, tiles[t]);
+    sweep_tile<T, EXACT, CPL, PF>(F, tiles[t]);
 }
 
 }  // namespace adf

@@ -7,33 +7,36 @@
 
 namespace adf {
 
-// Interior tiles on the main stream, the side list on the side stream, at
-// the same time (fork/join through events; both
-// streams default priority).
-template <typename T, bool EXACT, int CPL, int TY, int KIND>
-static void launch_kind(adf_solver* s, const Fused<T>& F, hipStream_t st) {
-    const int n = s->ntl[KIND];
+// The tiles of list LIST on stream st.
+template <typename T, bool EXACT, int CPL, int LIST>
+static void launch_list(adf_solver* s, const Fused<T>& F, hipStream_t st) {
+    const int n = s->ntl[LIST];
     if (!n) return;
     const int W = s->waves ? s->waves : 1;
     const dim3 g((unsigned)((n + W - 1) / W));
     if (s->pipeline)
-        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, TY, KIND, true>), g, dim3(64 * W), 0, st, F,
-                           (const uint2*)s->tiles[KIND], n);
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true>), g, dim3(64 * W), 0, st, F, (const uint2*)s->tiles[LIST], n);
     else
-        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, TY, KIND, false>), g, dim3(64 * W), 0, st, F,
-                           (const uint2*)s->tiles[KIND], n);
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, false>), g, dim3(64 * W), 0, st, F, (const uint2*)s->tiles[LIST], n);
 }
 
-template <typename T, bool EXACT, int CPL, int TY>
-static void launch_fused_ty(adf_solver* s, const Fused<T>& F, hipStream_t st) {
+// One launch over all tiles, or side and interior tiles as two launches
+// (on two streams of equal priority at the same time through fork/join
+// events, or one after the other).
+template <typename T, bool EXACT, int CPL>
+static void launch_fused_cpl(adf_solver* s, const Fused<T>& F, hipStream_t st) {
+    if (s->launch == ADF_SWEEP_ONE_LAUNCH) {
+        launch_list<T, EXACT, CPL, LIST_ALL>(s, F, st);
+        return;
+    }
     hipStream_t side = s->ctx->side;
-    const bool fork = s->ntl[LIST_SIDE] && s->ntl[LIST_INT] && s->concurrent;
+    const bool fork = s->ntl[LIST_SIDE] && s->ntl[LIST_INT] && s->launch == ADF_SWEEP_TWO_STREAMS;
     if (fork) {
         (void)hipEventRecord(s->ev_fork, st);
         (void)hipStreamWaitEvent(side, s->ev_fork, 0);
     }
-    launch_kind<T, EXACT, CPL, SWEEP_TY_SIDE, LIST_SIDE>(s, F, fork ? side : st);
-    launch_kind<T, EXACT, CPL, TY, LIST_INT>(s, F, st);
+    launch_list<T, EXACT, CPL, LIST_SIDE>(s, F, fork ? side : st);
+    launch_list<T, EXACT, CPL, LIST_INT>(s, F, st);
     if (fork) {
         (void)hipEventRecord(s->ev_join, side);
         (void)hipStreamWaitEvent(st, s->ev_join, 0);
@@ -42,38 +45,42 @@ static void launch_fused_ty(adf_solver* s, const Fused<T>& F, hipStream_t st) {
 
 template <typename T, bool EXACT>
 void launch_fused(adf_solver* s, const Fused<T>& F, int cpl, hipStream_t st) {
-    if (cpl == 4) launch_fused_ty<T, EXACT, 4, SWEEP_TY>(s, F, st);
-    else if (cpl == 2) launch_fused_ty<T, EXACT, 2, SWEEP_TY>(s, F, st);
-    else launch_fused_ty<T, EXACT, 1, SWEEP_TY>(s, F, st);
+    if (cpl == 4) launch_fused_cpl<T, EXACT, 4>(s, F, st);
+    else if (cpl == 2) launch_fused_cpl<T, EXACT, 2>(s, F, st);
+    else launch_fused_cpl<T, EXACT, 1>(s, F, st);
 }
 
 
-template <typename T, bool EXACT, int CPL, int TY, int KIND>
-static void batch_launch_kind(adf_batch* b, const adf_batch::Group& g, hipStream_t st) {
-    if (!g.n[KIND]) return;
+template <typename T, bool EXACT, int CPL, int LIST>
+static void batch_launch_list(adf_batch* b, const adf_batch::Group& g, hipStream_t st) {
+    if (!g.n[LIST]) return;
     const Fused<T>* fs = (const Fused<T>*)b->fs;
     const int pers = b->persistent ? 1 : 0;
     const dim3 blk(64 * g.waves);
     if (g.pf)
-        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, TY, KIND, true>), dim3(g.grid[KIND]), blk, 0, st, fs, b->cur,
-                           g.tiles[KIND], g.n[KIND], g.ctr[KIND], pers);
+        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, true>), dim3(g.grid[LIST]), blk, 0, st, fs, b->cur,
+                           g.tiles[LIST], g.n[LIST], g.ctr[LIST], pers);
     else
-        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, TY, KIND, false>), dim3(g.grid[KIND]), blk, 0, st, fs, b->cur,
-                           g.tiles[KIND], g.n[KIND], g.ctr[KIND], pers);
+        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, false>), dim3(g.grid[LIST]), blk, 0, st, fs, b->cur,
+                           g.tiles[LIST], g.n[LIST], g.ctr[LIST], pers);
 }
 
-template <typename T, bool EXACT, int CPL, int TY>
-static void batch_launch_ty(adf_batch* b, const adf_batch::Group& g, hipStream_t st) {
-    batch_launch_kind<T, EXACT, CPL, SWEEP_TY_SIDE, LIST_SIDE>(b, g, st);
-    batch_launch_kind<T, EXACT, CPL, TY, LIST_INT>(b, g, st);
+template <typename T, bool EXACT, int CPL>
+static void batch_launch_cpl(adf_batch* b, const adf_batch::Group& g, hipStream_t st) {
+    if (g.merged) {
+        batch_launch_list<T, EXACT, CPL, LIST_ALL>(b, g, st);
+    } else {
+        batch_launch_list<T, EXACT, CPL, LIST_SIDE>(b, g, st);
+        batch_launch_list<T, EXACT, CPL, LIST_INT>(b, g, st);
+    }
 }
 
 template <typename T, bool EXACT>
 void launch_fused_batch(adf_batch* b, hipStream_t st) {
     for (const auto& g : b->groups) {
-        if (g.cpl == 4) batch_launch_ty<T, EXACT, 4, SWEEP_TY>(b, g, st);
-        else if (g.cpl == 2) batch_launch_ty<T, EXACT, 2, SWEEP_TY>(b, g, st);
-        else batch_launch_ty<T, EXACT, 1, SWEEP_TY>(b, g, st);
+        if (g.cpl == 4) batch_launch_cpl<T, EXACT, 4>(b, g, st);
+        else if (g.cpl == 2) batch_launch_cpl<T, EXACT, 2>(b, g, st);
+        else batch_launch_cpl<T, EXACT, 1>(b, g, st);
     }
 }
 

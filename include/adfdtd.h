@@ -202,10 +202,16 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
  *                        (two plane buffers in registers); 0: one plane at
  *                        a time */
 #define ADF_OPT_PIPELINE 7
-/*   ADF_OPT_CONCURRENT   1 (default): general and interior tiles of the fused
- *                        kernel run as two launches on two streams at the
- *                        same time; 0: one after the other on one stream */
-#define ADF_OPT_CONCURRENT 8
+/*   ADF_OPT_SWEEP_LAUNCH how the fused kernel's tiles are launched:
+ *                        ADF_SWEEP_ONE_LAUNCH (default) one launch over all
+ *                        tiles; ADF_SWEEP_TWO_STREAMS side tiles (CPML slabs,
+ *                        boundary) and interior tiles as two launches on two
+ *                        streams at the same time; ADF_SWEEP_TWO_LAUNCHES
+ *                        the same two launches one after the other */
+#define ADF_OPT_SWEEP_LAUNCH 8
+#define ADF_SWEEP_ONE_LAUNCH 0
+#define ADF_SWEEP_TWO_STREAMS 1
+#define ADF_SWEEP_TWO_LAUNCHES 2
 int adf_solver_set_option(adf_solver* s, int option, int64_t value);
 
 /* Mean device time (ms) of one field-update launch and the number of

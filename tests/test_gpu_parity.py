@@ -19,12 +19,14 @@ pytestmark = pytest.mark.gpu
 TOL = {"f64": 1e-12, "ref": 1e-6, "fast": 1e-5}
 
 # Both kernel paths (two-pass P/V kernels and the fused z-marching sweep,
-# with several tile shapes, waves per workgroup and the pipelined interior
-# routine) must give the same results.
-PATHS = [("two-pass", 0, 0, 0, 0), ("fused", 0, 3, 1, 0), ("fused", 0, 0, 2, 1), ("fused", 0, 5, 4, 1)]
+# with several tile shapes, waves per workgroup, the pipelined and the
+# plane-at-a-time routines, and the three launch forms) must give the same
+# results.  (path, tile_y, block_z, waves, pipeline, launch)
+PATHS = [("two-pass", 0, 0, 0, 0, 0), ("fused", 0, 3, 1, 0, 0), ("fused", 0, 0, 2, 1, 1), ("fused", 0, 5, 4, 1, 0),
+         ("fused", 0, 2, 1, 1, 2)]
 
 
-@pytest.fixture(params=PATHS, ids=lambda p: f"{p[0]}-ty{p[1]}-kb{p[2]}-w{p[3]}-pf{p[4]}")
+@pytest.fixture(params=PATHS, ids=lambda p: f"{p[0]}-ty{p[1]}-kb{p[2]}-w{p[3]}-pf{p[4]}-l{p[5]}")
 def path(request):
     return request.param
 
@@ -40,7 +42,8 @@ def _compare(d, txs, cpml, mask, kind, field=None, gf_pair=None, path=("auto", 0
     dtype = np.float64 if kind == "f64" else np.float32
     prec = "ref" if kind == "ref" else "fast"
     f, g, cp = gpu_run(d, txs, cpml, mask, dtype, prec, field=field, gf=None if gf_pair is None else gf_pair[1],
-                       path=path[0], tile_y=path[1], block_z=path[2], waves=path[3], pipeline=path[4])
+                       path=path[0], tile_y=path[1], block_z=path[2], waves=path[3], pipeline=path[4],
+                       launch=path[5])
     tol = TOL[kind]
     errs = {
         "p_rec": rel_l2(g.rec[0], ref.tx["rec"][0]),
@@ -200,7 +203,7 @@ def test_inject_inner_sources_after_setup(kind, path):
     ref.run()
     cp = A.make_cpml(d, cpml.n, cpml.rcoef, cpml.fc, T)
     s = A.Solver(d, A.dtype_code(T, "ref" if kind == "ref" else "fast"), cp)
-    s.set_path(path[0], path[1], path[2], path[3], pipeline=path[4])
+    s.set_path(path[0], path[1], path[2], path[3], pipeline=path[4], launch=path[5])
     s.set_transceivers(g)  # zero sources at packing time
     s.set_trace(A._lib.ADF_SRC_P, p_src)
     s.set_trace(A._lib.ADF_VN_SRC, vn)

@@ -28,7 +28,7 @@ def test_tune_solver_cached_and_result_invariant(tmp_path):
     ref.run()
     s, g = _solver(case)
     shape = autotune.tune_solver(s, steps=6, path=path)
-    keys = ("tile_y", "block_z", "block_z_gen", "waves", "pipeline", "concurrent")
+    keys = ("tile_y", "block_z", "block_z_gen", "waves", "pipeline", "launch")
     assert {k: shape[k] for k in keys} in autotune.SOLVER_CANDIDATES
     data = json.load(open(path))
     assert len(data) == 1 and next(iter(data)).startswith("solver:")
