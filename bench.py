@@ -16,6 +16,7 @@ restatement of the reference) timed on the host's cores as the CPU baseline.
 
 Modes for profiling (see profiles/):
   --profile-steps N   run N steps only (no CPU leg), for rocprofv3
+  --profile-batch N   run the 64 x 128^3 batched leg for N steps only
   --calibrate-copy    time a plain 1 GiB device copy (PMC scale factor)
 """
 from __future__ import annotations
@@ -230,11 +231,13 @@ def batched_leg(n_solves, sizes, nt, tune=True):
     cells = sum(sp["d"].nx * sp["d"].ny * sp["d"].nz for sp in specs)
     peak = float(np.abs(sol[0].get_trace(0)).max())
     del b, sol
+    traffic = read_traffic("sweep_batch_128_f32.md") if sizes == [128] and n_solves == 64 else None
     return {"solves": len(specs), "grids": sorted({sp["d"].nx for sp in specs}), "steps": nt,
             "run_s": round(run_s, 4), "wall_s": round(wall, 4), "solves_per_s": round(len(specs) / run_s, 3),
             "ms_per_batched_step": round(ms, 4), "glups": round(cells / (ms * 1e-3) / 1e9, 2),
             "algorithmic_GBps": round(byts / (ms * 1e-3) / 1e9, 1),
             "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "tiling": shape,
+            "sweep_traffic_bytes": traffic, "algorithmic_bytes_per_step": byts,
             "p_rec_peak_solve0": peak}
 
 
@@ -380,6 +383,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=300)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=0)
+    ap.add_argument("--profile-batch", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-served", action="store_true")
@@ -393,6 +397,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.profile_batch:
+        print(json.dumps(batched_leg(args.batch_solves, [128], args.profile_batch)))
         return
     if args.profile_steps:
         args.steps, args.warmup, args.fp64_steps = args.profile_steps, 2, 0
@@ -424,7 +431,8 @@ def main():
         "hbm_frac_step": round(b32 / (f32["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_sweep general + interior tiles (field-update sweep)", "algorithmic_bytes_per_launch": b32,
+                     "kernel": "k_sweep (field-update sweep of one step: one launch over interior and side tiles)",
+                     "algorithmic_bytes_per_launch": b32,
                      "avg_launch_ms": round(f32["sweep_ms"], 5), "launches": f32["sweep_launches"],
                      "tiling": f32["tiling"]},
     }
