@@ -68,81 +68,56 @@ __global__ __launch_bounds__(256) void k_sweep_batch(const Fused<T>* __restrict_
     }
 }
 
-// entry = solve << 20 | injector << 19 | transceiver
+// entry = solve << 20 | injector << 19 | transceiver; 8 lanes per entry
+// (tx_corners)
 template <typename T>
-__global__ void k_tx_p_batch(const TxDev<T>* __restrict__ td, const int* __restrict__ it0s,
-                             const int* __restrict__ cur, const uint32_t* __restrict__ ent, int n) {
+__global__ __launch_bounds__(256) void k_tx_p_batch(const TxDev<T>* __restrict__ td, const int* __restrict__ it0s,
+                                                    const int* __restrict__ cur, const uint32_t* __restrict__ ent,
+                                                    int n) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const uint32_t e = ent[t];
+    const int g = t >> 3, q = t & 7, g0 = (int)(threadIdx.x & 63) & ~7;
+    const bool live = g < n;
+    const uint32_t e = live ? ent[g] : 0u;
     const int s = (int)(e >> 20);
     const int is = (int)(e & 0x7ffffu);
     const TxDev<T> D = td[s];
     const Txs<T>& X = D.X;
     T* P = D.fb[cur[s] ^ 1];
-    const Corners C = corners_of(X.idx + 3 * is, D.nx, D.ny, D.nz);
-    if (!C.ok) return;
-    T f[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) f[q] = P[C.c[q]];
     const size_t o = (size_t)is * X.nt + it0s[s];
-    const T* w8 = X.w + 8 * is;
-    if ((e >> 19) & 1u) {
-        interp_apply<T>(P, C, w8, f, &X.rec[o], X.src[o]);
-    } else {
-        T r = X.rec[o];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) r = r + w8[q] * f[q];
-        X.rec[o] = r;
-    }
+    tx_corners<T>(P, X.idx + 3 * is, D.nx, D.ny, D.nz, X.w + 8 * is, X.rec + o, X.src + o, ((e >> 19) & 1u) != 0,
+                  live, q, g0);
 }
 
+// 32 lanes per entry: channels x, y, z on lanes 0-7, 8-15, 16-23 (k_tx_v).
 template <typename T>
-__global__ void k_tx_v_batch(const TxDev<T>* __restrict__ td, const int* __restrict__ it0s,
-                             const int* __restrict__ cur, const uint32_t* __restrict__ ent, int n) {
+__global__ __launch_bounds__(256) void k_tx_v_batch(const TxDev<T>* __restrict__ td, const int* __restrict__ it0s,
+                                                    const int* __restrict__ cur, const uint32_t* __restrict__ ent,
+                                                    int n) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const uint32_t e = ent[t];
+    const int g = t >> 5, ch = (t >> 3) & 3, q = t & 7;
+    const int lw = (int)(threadIdx.x & 63), g32 = lw & ~31, g0 = lw & ~7;
+    const bool any = g < n;
+    const bool live = any && ch < 3;
+    const uint32_t e = any ? ent[g] : 0u;
     const int s = (int)(e >> 20);
     const int is = (int)(e & 0x7ffffu);
-    const bool inj = (e >> 19) & 1u;
+    const bool inj = ((e >> 19) & 1u) != 0;
     const TxDev<T> D = td[s];
     const Txs<T>& X = D.X;
-    T* Fb = D.fb[cur[s] ^ 1];
     const int ntx = X.ntx, nt = X.nt, it0 = it0s[s];
     const size_t o = (size_t)is * nt + it0;
     const size_t cs = (size_t)ntx * nt;
-    Corners C[3];
-    T f[3][8];
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-        C[ch] = corners_of(X.idx + 3 * (is + ntx * (ch + 1)), D.nx, D.ny, D.nz);
-        T* Fc = Fb + (size_t)(ch + 1) * D.n3;
-        if (C[ch].ok) {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) f[ch][q] = Fc[C[ch].c[q]];
-        }
-    }
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-        if (!C[ch].ok) continue;
-        T* Fc = Fb + (size_t)(ch + 1) * D.n3;
-        const T* w8 = X.w + 8 * (is + ntx * (ch + 1));
-        if (inj) {
-            interp_apply<T>(Fc, C[ch], w8, f[ch], &X.rec[o + cs * (ch + 1)], X.src[o + cs * (ch + 1)]);
-        } else {
-            T r = X.rec[o + cs * (ch + 1)];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) r = r + w8[q] * f[ch][q];
-            X.rec[o + cs * (ch + 1)] = r;
-        }
-    }
+    const int c1 = (ch < 3 ? ch : 2) + 1;
+    T* F = D.fb[cur[s] ^ 1] + (size_t)c1 * D.n3;
+    const T r = tx_corners<T>(F, X.idx + 3 * (is + ntx * c1), D.nx, D.ny, D.nz, X.w + 8 * (is + ntx * c1),
+                              X.rec + o + cs * c1, X.src + o + cs * c1, inj, live, q, g0);
+    const T rx = __shfl(r, g32, 64), ry = __shfl(r, g32 + 8, 64), rz = __shfl(r, g32 + 16, 64);
+    if (!any) return;
     const T* d = X.dir + 4 * is;
-    X.vn_rec[o] = X.rec[o + cs] * d[1] + X.rec[o + 2 * cs] * d[2] + X.rec[o + 3 * cs] * d[3];
-    if (inj && it0 + 1 < nt) {  // vn2v! into it+1 (forward.jl:130-132)
+    if (lw == g32) X.vn_rec[o] = rx * d[1] + ry * d[2] + rz * d[3];
+    if (inj && it0 + 1 < nt && live && q == 0) {  // vn2v! into it+1 (forward.jl:130-132)
         const T vn = X.vn_src[o + 1];
-#pragma unroll
-        for (int ch = 1; ch <= 3; ++ch) X.src[o + 1 + cs * ch] = X.src[o + 1 + cs * ch] + vn * d[ch];
+        X.src[o + 1 + cs * c1] = X.src[o + 1 + cs * c1] + vn * d[c1];
     }
 }
 

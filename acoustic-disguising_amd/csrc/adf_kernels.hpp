@@ -248,106 +248,79 @@ struct Txs {
     int ntx, nt;
 };
 
-// derivatives.jl:30-60 for one transceiver on one channel: corners in the
-// reference order (i fastest), record then inject per corner.  The eight
-// corners are distinct cells, so all of them are loaded before any is
-// written (same values as the reference's corner-by-corner loop) and the
-// loads overlap instead of forming a load->store chain.
-struct Corners {
-    int c[8];
-    bool ok;
-};
-
-__device__ __forceinline__ Corners corners_of(const int32_t* id3, int nx, int ny, int nz) {
-    Corners C;
-    const int I = id3[0], J = id3[1], K = id3[2];
-    C.ok = I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz;
-    const int base = (I - 1) + nx * ((J - 1) + ny * (K - 1));
+// One transceiver on one channel over 8 lanes of a wave (q = corner, i
+// fastest; derivatives.jl:30-60): each lane loads its corner and weight,
+// the record is summed in the reference's corner order (every lane runs the
+// same sequence of shuffles, so every lane holds the sum), then each lane
+// injects into its corner.  The eight corners are distinct cells, so this
+// equals the reference's record-then-inject loop value for value.  `live`
+// lanes belong to a real entry; all lanes must call this (shuffles).
+// Returns the new record (the old one when the stencil is skipped).
+template <typename T>
+__device__ __forceinline__ T tx_corners(T* F, const int32_t* id3, int nx, int ny, int nz, const T* w8, T* rec,
+                                        const T* src, bool inj, bool live, int q, int g0) {
+    int I = 0, J = 0, K = 0;
+    if (live) { I = id3[0]; J = id3[1]; K = id3[2]; }
+    const bool ok = live && I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz;
+    const int c = (I - 1) + nx * ((J - 1) + ny * (K - 1)) + (q & 1) + nx * ((q >> 1) & 1) + nx * ny * (q >> 2);
+    const T f = ok ? F[c] : (T)0;
+    const T w = ok ? w8[q] : (T)0;
+    const T r0 = live ? *rec : (T)0;
+    const T sv = (ok && inj) ? *src : (T)0;
+    const T prod = w * f;
+    T r = r0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) C.c[q] = base + (q & 1) + nx * ((q >> 1) & 1) + nx * ny * (q >> 2);
-    return C;
+    for (int qq = 0; qq < 8; ++qq) r = r + __shfl(prod, g0 + qq, 64);
+    if (!ok) r = r0;
+    if (ok && q == 0) *rec = r;
+    if (ok && inj) F[c] = f + w * sv;
+    return r;
 }
 
+// Pressure transceivers of one conflict-free level (forward.jl:92), 8 lanes
+// per transceiver.  order entries carry the injector flag in bit 31:
+// transceivers whose p source is zero for the whole run only record.
 template <typename T>
-__device__ __forceinline__ void interp_apply(T* F, const Corners& C, const T* w8, const T (&f)[8], T* rec, T src) {
-    T r = *rec;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) r = r + w8[q] * f[q];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) F[C.c[q]] = f[q] + w8[q] * src;
-    *rec = r;
-}
-
-// Pressure transceivers of one conflict-free level (forward.jl:92).  order
-// entries carry the injector flag in bit 31: transceivers whose p source is
-// zero for the whole run only record (adding w*0 leaves every value as is).
-template <typename T>
-__global__ void k_tx_p(T* P, int nx, int ny, int nz, Txs<T> X, const uint32_t* order, int n, int it0) {
+__global__ __launch_bounds__(256) void k_tx_p(T* P, int nx, int ny, int nz, Txs<T> X, const uint32_t* order, int n,
+                                              int it0) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const uint32_t e = order[t];
+    const int g = t >> 3, q = t & 7, g0 = (int)(threadIdx.x & 63) & ~7;
+    const bool live = g < n;
+    const uint32_t e = live ? order[g] : 0u;
     const int is = (int)(e & 0x7fffffffu);
-    const Corners C = corners_of(X.idx + 3 * is, nx, ny, nz);
-    if (!C.ok) return;
-    T f[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) f[q] = P[C.c[q]];
     const size_t o = (size_t)is * X.nt + it0;
-    if (e >> 31) {
-        interp_apply<T>(P, C, X.w + 8 * is, f, &X.rec[o], X.src[o]);
-    } else {
-        const T* w8 = X.w + 8 * is;
-        T r = X.rec[o];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) r = r + w8[q] * f[q];
-        X.rec[o] = r;
-    }
+    tx_corners<T>(P, X.idx + 3 * is, nx, ny, nz, X.w + 8 * is, X.rec + o, X.src + o, (e >> 31) != 0, live, q, g0);
 }
 
 // Velocity transceivers of one level (forward.jl:111-115), then v2vn!
 // (forward.jl:120) and, when no GF map is active, vn2v! into it+1
-// (forward.jl:130-132) for the same transceiver.
+// (forward.jl:130-132) for the same transceiver.  32 lanes per
+// transceiver: channel x, y, z on lanes 0-7, 8-15, 16-23.
 template <typename T>
-__global__ void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, Txs<T> X, const uint32_t* order,
-                       int n, int it0, int do_vn2v) {
+__global__ __launch_bounds__(256) void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, Txs<T> X,
+                                              const uint32_t* order, int n, int it0, int do_vn2v) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const uint32_t e = order[t];
+    const int g = t >> 5, ch = (t >> 3) & 3, q = t & 7;
+    const int lw = (int)(threadIdx.x & 63), g32 = lw & ~31, g0 = lw & ~7;
+    const bool any = g < n;
+    const bool live = any && ch < 3;
+    const uint32_t e = any ? order[g] : 0u;
     const int is = (int)(e & 0x7fffffffu);
     const bool inj = (e >> 31) != 0;
     const int ntx = X.ntx, nt = X.nt;
     const size_t o = (size_t)is * nt + it0;
     const size_t cs = (size_t)ntx * nt;  // channel stride of src/rec
-    T* Fc[3] = {Vx, Vy, Vz};
-    Corners C[3];
-    T f[3][8];
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-        C[ch] = corners_of(X.idx + 3 * (is + ntx * (ch + 1)), nx, ny, nz);
-        if (C[ch].ok) {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) f[ch][q] = Fc[ch][C[ch].c[q]];
-        }
-    }
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-        if (!C[ch].ok) continue;
-        const T* w8 = X.w + 8 * (is + ntx * (ch + 1));
-        if (inj) {
-            interp_apply<T>(Fc[ch], C[ch], w8, f[ch], &X.rec[o + cs * (ch + 1)], X.src[o + cs * (ch + 1)]);
-        } else {
-            T r = X.rec[o + cs * (ch + 1)];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) r = r + w8[q] * f[ch][q];
-            X.rec[o + cs * (ch + 1)] = r;
-        }
-    }
+    const int c1 = (ch < 3 ? ch : 2) + 1;
+    T* F = c1 == 1 ? Vx : (c1 == 2 ? Vy : Vz);
+    const T r = tx_corners<T>(F, X.idx + 3 * (is + ntx * c1), nx, ny, nz, X.w + 8 * (is + ntx * c1),
+                              X.rec + o + cs * c1, X.src + o + cs * c1, inj, live, q, g0);
+    const T rx = __shfl(r, g32, 64), ry = __shfl(r, g32 + 8, 64), rz = __shfl(r, g32 + 16, 64);
+    if (!any) return;
     const T* d = X.dir + 4 * is;
-    X.vn_rec[o] = X.rec[o + cs] * d[1] + X.rec[o + 2 * cs] * d[2] + X.rec[o + 3 * cs] * d[3];
-    if (do_vn2v && inj && it0 + 1 < nt) {
+    if (lw == g32) X.vn_rec[o] = rx * d[1] + ry * d[2] + rz * d[3];
+    if (do_vn2v && inj && it0 + 1 < nt && live && q == 0) {
         const T vn = X.vn_src[o + 1];
-#pragma unroll
-        for (int ch = 1; ch <= 3; ++ch) X.src[o + 1 + cs * ch] = X.src[o + 1 + cs * ch] + vn * d[ch];
+        X.src[o + 1 + cs * c1] = X.src[o + 1 + cs * c1] + vn * d[c1];
     }
 }
 

@@ -576,7 +576,7 @@ static void launch_tx_p(adf_solver* s, const Solve<T>& S, const Txs<T>& X, int i
     for (size_t L = 0; L + 1 < s->lev_p.size(); ++L) {
         const int n = s->lev_p[L + 1] - s->lev_p[L];
         if (n > 0)
-            hipLaunchKernelGGL((k_tx_p<T>), dim3((n + 255) / 256), dim3(256), 0, st, S.P, S.nx, S.ny, S.nz, X,
+            hipLaunchKernelGGL((k_tx_p<T>), dim3((8 * n + 255) / 256), dim3(256), 0, st, S.P, S.nx, S.ny, S.nz, X,
                                s->order_p + s->lev_p[L], n, it0);
     }
 }
@@ -590,7 +590,7 @@ static void launch_after_v(adf_solver* s, const Solve<T>& S, const Txs<T>& X, in
         for (size_t L = 0; L + 1 < s->lev_v.size(); ++L) {
             const int n = s->lev_v[L + 1] - s->lev_v[L];
             if (n > 0)
-                hipLaunchKernelGGL((k_tx_v<T>), dim3((n + 255) / 256), dim3(256), 0, st, S.Vx, S.Vy, S.Vz, S.nx, S.ny,
+                hipLaunchKernelGGL((k_tx_v<T>), dim3((32 * n + 255) / 256), dim3(256), 0, st, S.Vx, S.Vy, S.Vz, S.nx, S.ny,
                                    S.nz, X, s->order_v + s->lev_v[L], n, it0, do_vn2v);
         }
         if (s->gf_kind) {
@@ -1181,7 +1181,7 @@ static void batch_step(adf_batch* b, hipStream_t st) {
     for (size_t L = 0; L + 1 < b->txp_lev.size(); ++L) {
         const int n = b->txp_lev[L + 1] - b->txp_lev[L];
         if (n > 0)
-            hipLaunchKernelGGL((k_tx_p_batch<T>), dim3((n + 255) / 256), dim3(256), 0, st, td, b->it0, b->cur,
+            hipLaunchKernelGGL((k_tx_p_batch<T>), dim3((8 * n + 255) / 256), dim3(256), 0, st, td, b->it0, b->cur,
                                b->txp + b->txp_lev[L], n);
     }
     if (b->n_fix)
@@ -1190,7 +1190,7 @@ static void batch_step(adf_batch* b, hipStream_t st) {
     for (size_t L = 0; L + 1 < b->txv_lev.size(); ++L) {
         const int n = b->txv_lev[L + 1] - b->txv_lev[L];
         if (n > 0)
-            hipLaunchKernelGGL((k_tx_v_batch<T>), dim3((n + 255) / 256), dim3(256), 0, st, td, b->it0, b->cur,
+            hipLaunchKernelGGL((k_tx_v_batch<T>), dim3((32 * n + 255) / 256), dim3(256), 0, st, td, b->it0, b->cur,
                                b->txv + b->txv_lev[L], n);
     }
     if (b->n_mirror)
