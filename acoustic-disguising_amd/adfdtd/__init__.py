@@ -16,5 +16,6 @@ from .batch import Batch
 from .server import Server
 from .autotune import tune_batch, tune_solver
 from .kh import kh_analytic_accumulate
+from .greens import impulsive_gfs, scatterer_params
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -253,6 +253,18 @@ class Solver:
         self.set_option(L.ADF_OPT_PIPELINE, pipeline)
         self.set_option(L.ADF_OPT_SWEEP_LAUNCH, launch)
 
+    def gf_resample(self, first: int, n_rec: int, timeshift: float, nt_gf: int):
+        """The per-run tail of impulsive_gfs (src/greens.jl:166-191) on the
+        device: p and vn traces of transceivers first..first+n_rec-1
+        (0-based), denoised and resampled onto range(0, tmax; length =
+        nt_gf) from range(-timeshift, tmax - timeshift; length = nt) ->
+        two float32 arrays (n_rec, nt_gf) (Julia (nt_gf, n_rec))."""
+        out_p = np.empty((n_rec, nt_gf), np.float32)
+        out_vn = np.empty((n_rec, nt_gf), np.float32)
+        L.check(self.lib.adf_solver_gf_resample(self.h, first, n_rec, float(timeshift), nt_gf, L.ptr(out_p),
+                                                L.ptr(out_vn)), "gf_resample")
+        return out_p, out_vn
+
     def sweep_stats(self):
         """(mean ms of one field-update launch, launches) of the last run with
         set_option(ADF_OPT_TIME_SWEEP, 1)."""

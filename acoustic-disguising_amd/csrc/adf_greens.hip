@@ -1,0 +1,145 @@
+// impulsive_gfs post-processing of one finished run on the device
+// (include/adfdtd.h adf_solver_gf_resample; src/greens.jl:166-191):
+//
+//   A      = receiver traces p and vn of transceivers first .. first+n_rec-1
+//            (greens.jl:171-174; first = 1 drops the source's self-record)
+//   A     := 0 where |A| < 1e-4 max_t A, the signed maximum of each trace
+//            (greens.jl:176-177)
+//   A_gf   = resample_time(A, t_shifted, t_gf; kind = :cubic)
+//            (greens.jl:183; src/resample.jl:35-77): the cubic B-spline with
+//            Line(OnGrid()) ends (the natural spline) through A on
+//            t_shifted = range(-timeshift, tmax - timeshift; length = nt),
+//            Flat outside, sampled on t_gf = range(0, tmax; length =
+//            nt_gf), times step(t_gf) / step(t_shifted), stored as float32.
+//
+// One thread per trace solves the spline system along time (Thomas
+// algorithm in double, the shared pivot sequence computed on the host);
+// one thread per output sample evaluates the four B-spline taps.  Along the
+// receiver and channel axes the reference's 3-D interpolant is evaluated
+// only at integer nodes, where the interpolant equals the data.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/adfdtd.h"
+#include "adf_internal.hpp"
+
+using namespace adf;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// Denoise trace col and solve for its spline coefficients c_0 .. c_{n+1}
+// (rows (c_{i-1} + 4 c_i + c_{i+1}) / 6 = y_i, c_1 = y_1, c_n = y_n, zero
+// second difference at both ends).  cp: pivots 1/(4 - cp_{i-1}), cp_0 = 1/4.
+template <typename T>
+__global__ void k_gf_coef(const T* __restrict__ rec_p, const T* __restrict__ rec_vn, int nt, int first, int n_rec,
+                          const double* __restrict__ cp, double* __restrict__ coef) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= 2 * n_rec) return;
+    const int which = col / n_rec, r = col - which * n_rec;
+    const T* a = (which ? rec_vn : rec_p) + (size_t)(first + r) * nt;
+    double* c = coef + (size_t)col * (nt + 2);
+    double mx = (double)a[0];
+    for (int k = 1; k < nt; ++k) mx = fmax(mx, (double)a[k]);
+    const double thr = 1e-4 * mx;
+    auto y = [&](int k) {  // denoised sample k (0-based)
+        const double v = (double)a[k];
+        return fabs(v) < thr ? 0.0 : v;
+    };
+    const int n = nt;
+    if (n == 1) {
+        c[0] = c[1] = c[2] = y(0);
+        return;
+    }
+    c[1] = y(0);
+    c[n] = y(n - 1);
+    const int m = n - 2;  // unknowns c_2 .. c_{n-1}, forward sweep into c
+    for (int i = 0; i < m; ++i) {
+        double rhs = 6.0 * y(i + 1);
+        if (i == 0) rhs -= c[1];
+        if (i == m - 1) rhs -= c[n];
+        c[2 + i] = i == 0 ? rhs / 4.0 : (rhs - c[1 + i]) / (4.0 - cp[i - 1]);
+    }
+    for (int i = m - 2; i >= 0; --i) c[2 + i] = c[2 + i] - cp[i] * c[3 + i];
+    c[0] = 2.0 * c[1] - c[2];
+    c[n + 1] = 2.0 * c[n] - c[n - 1];
+}
+
+// out[col][k] = scale * s_col(x_k), x_k = 1 + (t_out0 + k dt_out - t_in0) / dt_in
+// clamped to [1, n].
+__global__ void k_gf_eval(const double* __restrict__ coef, int n, int ncol, double t_in0, double dt_in,
+                          double t_out0, double dt_out, int n_out, double scale, float* __restrict__ out) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)ncol * n_out) return;
+    const int col = (int)(g / n_out), k = (int)(g - (int64_t)col * n_out);
+    const double* c = coef + (size_t)col * (n + 2);
+    double x = 1.0 + ((t_out0 + (double)k * dt_out) - t_in0) / dt_in;
+    x = fmin(fmax(x, 1.0), (double)n);
+    int i = (int)floor(x);
+    if (i > n - 1) i = n - 1 > 1 ? n - 1 : 1;
+    const double d = x - (double)i, e = 1.0 - d;
+    const double w0 = e * e * e / 6.0;
+    const double w1 = 2.0 / 3.0 - d * d + d * d * d / 2.0;
+    const double w2 = 2.0 / 3.0 - e * e + e * e * e / 2.0;
+    const double w3 = d * d * d / 6.0;
+    auto cc = [&](int q) { return c[q < 0 ? 0 : (q > n + 1 ? n + 1 : q)]; };
+    const double v = w0 * cc(i - 1) + w1 * cc(i) + w2 * cc(i + 1) + w3 * cc(i + 2);
+    out[g] = (float)(v * scale);
+}
+
+}  // namespace
+
+extern "C" int adf_solver_gf_resample(adf_solver* s, int64_t first, int64_t n_rec, double timeshift, int64_t nt_gf,
+                                      float* out_p, float* out_vn) {
+    if (!s) return fail(ADF_EINVAL, "adf_solver_gf_resample: null solver");
+    if (first < 0 || n_rec < 1 || first + n_rec > s->ntx)
+        return fail(ADF_EINVAL, "adf_solver_gf_resample: transceivers [first, first + n_rec) outside the solver's list");
+    if (nt_gf < 1 || !out_p || !out_vn) return fail(ADF_EINVAL, "adf_solver_gf_resample: bad nt_gf or output");
+    const int64_t nt = s->dom.nt;
+    if (2 * n_rec * (nt + 2) > ((int64_t)1 << 31) || 2 * n_rec * nt_gf > ((int64_t)1 << 31))
+        return fail(ADF_EINVAL, "adf_solver_gf_resample: too many samples");
+    HIPCHK(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    // pivots of the inner tridiagonal system (the same for every trace)
+    std::vector<double> cp((size_t)std::max<int64_t>(nt - 2, 1));
+    cp[0] = 1.0 / 4.0;
+    for (size_t i = 1; i < cp.size(); ++i) cp[i] = 1.0 / (4.0 - cp[i - 1]);
+    DevBuf dcp, dcoef, dout;
+    HIPCHK(hipMalloc(&dcp.p, cp.size() * 8));
+    HIPCHK(hipMalloc(&dcoef.p, (size_t)2 * n_rec * (nt + 2) * 8));
+    HIPCHK(hipMalloc(&dout.p, (size_t)2 * n_rec * nt_gf * 4));
+    HIPCHK(hipMemcpyAsync(dcp.p, cp.data(), cp.size() * 8, hipMemcpyHostToDevice, st));
+    const int ncol = (int)(2 * n_rec);
+    const void* rp = solver_trace_ptr(s, ADF_REC_P);
+    const void* rv = solver_trace_ptr(s, ADF_VN_REC);
+    if (s->dtype == ADF_F64)
+        hipLaunchKernelGGL((k_gf_coef<double>), dim3((ncol + 63) / 64), dim3(64), 0, st, (const double*)rp,
+                           (const double*)rv, (int)nt, (int)first, (int)n_rec, (const double*)dcp.p, (double*)dcoef.p);
+    else
+        hipLaunchKernelGGL((k_gf_coef<float>), dim3((ncol + 63) / 64), dim3(64), 0, st, (const float*)rp,
+                           (const float*)rv, (int)nt, (int)first, (int)n_rec, (const double*)dcp.p, (double*)dcoef.p);
+    // t_shifted = range(-timeshift, tmax - timeshift; length = nt),
+    // t_gf = range(0, tmax; length = nt_gf)
+    const double tmax = s->dom.tmax;
+    const double dt_in = nt > 1 ? tmax / (double)(nt - 1) : 1.0;
+    const double dt_out = nt_gf > 1 ? tmax / (double)(nt_gf - 1) : 1.0;
+    const double scale = dt_out / dt_in;
+    const int64_t tot = (int64_t)ncol * nt_gf;
+    hipLaunchKernelGGL(k_gf_eval, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, (const double*)dcoef.p,
+                       (int)nt, ncol, -timeshift, dt_in, 0.0, dt_out, (int)nt_gf, scale, (float*)dout.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out_p, dout.p, (size_t)n_rec * nt_gf * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(out_vn, (float*)dout.p + (size_t)n_rec * nt_gf, (size_t)n_rec * nt_gf * 4,
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return ADF_OK;
+}

@@ -262,6 +262,23 @@ int adf_batch_set_option(adf_batch* b, int option, int64_t value);
 /* Mean device time per batched step of the last adf_batch_run (ms). */
 int adf_batch_last_step_ms(adf_batch* b, double* ms);
 
+/* ------------------------------------------- impulsive_gfs post-processing
+ *
+ * The per-run tail of impulsive_gfs (src/greens.jl:166-191) on the device,
+ * for a solver whose run has finished: the p and vn receiver traces of
+ * transceivers first .. first+n_rec-1 (0-based; first = 1 drops the
+ * source's self-record as greens.jl:171-174 does), denoised (a sample is
+ * zeroed where |A| < 1e-4 * the trace's signed maximum over time,
+ * greens.jl:176-177) and resampled (resample_time(A, t_shifted, t_gf;
+ * kind = :cubic), greens.jl:183, src/resample.jl:35-77: the cubic B-spline
+ * with Line(OnGrid()) ends through A on t_shifted = range(-timeshift,
+ * tmax - timeshift; length = nt), Flat outside, sampled on t_gf =
+ * range(0, tmax; length = nt_gf) and scaled by step(t_gf)/step(t_shifted)).
+ * out_p / out_vn: float32 (nt_gf, n_rec) column-major each, the layout of
+ * the reference's p_p[:, :, iSrc] / v_p[:, :, iSrc] slices. */
+int adf_solver_gf_resample(adf_solver* s, int64_t first, int64_t n_rec, double timeshift, int64_t nt_gf,
+                           float* out_p, float* out_vn);
+
 /* --------------------------------------------------------- post-hoc KH
  *
  * Analytic Kirchhoff-Helmholtz kernels evaluated on the device and

@@ -373,3 +373,87 @@ def set_threads(n: int) -> None:
         gomp.omp_set_num_threads(C.c_int(n))
     except OSError:
         pass
+
+
+# --------------------------------------------- impulsive_gfs post-processing
+#
+# Plain numpy (float64) restatements, checked against scipy's natural cubic
+# spline and closed forms in tests/test_oracle_pins.py.
+
+def gf_denoise(A):
+    """src/greens.jl:176-177: per trace (last axis = time here), zero every
+    sample with |A| < 1e-4 * max_t A, where the maximum is the SIGNED
+    maximum over time (``maximum(A, dims = 1)``)."""
+    A = np.array(A, dtype=np.float64, copy=True)
+    mx = A.max(axis=-1, keepdims=True)
+    A[np.abs(A) < 1e-4 * mx] = 0.0
+    return A
+
+
+def natural_bspline_coefficients(y):
+    """Coefficients c_0..c_{n+1} of the cubic B-spline interpolant of y at
+    the nodes 1..n with ``Line(OnGrid())`` boundaries (Interpolations.jl's
+    ``BSpline(Cubic(Line(OnGrid())))``, used by src/resample.jl:35-40): the
+    rows (c_{i-1} + 4 c_i + c_{i+1}) / 6 = y_i plus a zero second difference
+    c_0 - 2 c_1 + c_2 = 0 = c_{n-1} - 2 c_n + c_{n+1} at both ends (zero
+    second derivative at the end nodes: the natural spline).  The end rows
+    give c_1 = y_1 and c_n = y_n; the inner system is tridiagonal (Thomas
+    algorithm along the last axis)."""
+    y = np.asarray(y, dtype=np.float64)
+    n = y.shape[-1]
+    c = np.zeros(y.shape[:-1] + (n + 2,))
+    if n == 1:
+        c[..., :] = y[..., :1]
+        return c
+    c[..., 1] = y[..., 0]
+    c[..., n] = y[..., n - 1]
+    m = n - 2  # unknowns c_2..c_{n-1}
+    if m > 0:
+        rhs = 6.0 * y[..., 1:n - 1].copy()
+        rhs[..., 0] -= c[..., 1]
+        rhs[..., m - 1] -= c[..., n]
+        cp = np.zeros(m)
+        dp = np.zeros(y.shape[:-1] + (m,))
+        cp[0] = 1.0 / 4.0
+        dp[..., 0] = rhs[..., 0] / 4.0
+        for i in range(1, m):
+            den = 4.0 - cp[i - 1]
+            cp[i] = 1.0 / den
+            dp[..., i] = (rhs[..., i] - dp[..., i - 1]) / den
+        x = np.zeros_like(dp)
+        x[..., m - 1] = dp[..., m - 1]
+        for i in range(m - 2, -1, -1):
+            x[..., i] = dp[..., i] - cp[i] * x[..., i + 1]
+        c[..., 2:n] = x
+    c[..., 0] = 2.0 * c[..., 1] - c[..., 2]
+    c[..., n + 1] = 2.0 * c[..., n] - c[..., n - 1]
+    return c
+
+
+def bspline_eval(c, xq):
+    """Evaluate the cubic B-spline with coefficients c (last axis, nodes
+    1..n at c[1..n]) at index coordinates xq, clamped to [1, n] (``Flat``
+    extrapolation)."""
+    c = np.asarray(c, dtype=np.float64)
+    n = c.shape[-1] - 2
+    xq = np.clip(np.asarray(xq, dtype=np.float64), 1.0, float(n))
+    i = np.minimum(np.floor(xq).astype(np.int64), max(n - 1, 1))
+    d = xq - i
+    e = 1.0 - d
+    w0 = e * e * e / 6.0
+    w1 = 2.0 / 3.0 - d * d + d * d * d / 2.0
+    w2 = 2.0 / 3.0 - e * e + e * e * e / 2.0
+    w3 = d * d * d / 6.0
+    g = lambda k: np.take(c, np.clip(i + k, 0, n + 1), axis=-1)
+    return w0 * g(-1) + w1 * g(0) + w2 * g(1) + w3 * g(2)
+
+
+def resample_time(A, t_in0, dt_in, t_out0, dt_out, n_out, scale_amplitude=True):
+    """src/resample.jl:35-77 (kind = :cubic) along the last axis: the
+    natural cubic B-spline through A on t_in = t_in0 + (k-1) dt_in, Flat
+    outside, evaluated at t_out = t_out0 + (k-1) dt_out, times
+    dt_out / dt_in."""
+    c = natural_bspline_coefficients(A)
+    xq = 1.0 + (t_out0 + np.arange(n_out) * dt_out - t_in0) / dt_in
+    out = bspline_eval(c, xq)
+    return out * (dt_out / dt_in) if scale_amplitude else out

@@ -430,3 +430,57 @@ def test_direct_kh_delta_kernel():
     out = O.direct_kh_accumulate(np.zeros((2, 12)), K, inp, alpha=2.0)
     np.testing.assert_array_equal(out[1], 2 * inp[2])
     assert not out[0].any()
+
+
+# ------------------------------------------- impulsive_gfs post-processing
+
+
+def test_resample_is_the_natural_cubic_spline():
+    """resample_time(kind = :cubic) uses BSpline(Cubic(Line(OnGrid())))
+    (src/resample.jl:35-40): the C2 cubic with zero second derivative at the
+    end nodes.  scipy's CubicSpline(bc_type="natural") is an independent
+    implementation of that interpolant."""
+    from scipy.interpolate import CubicSpline
+
+    rng = np.random.default_rng(11)
+    y = rng.standard_normal((3, 57))
+    c = O.natural_bspline_coefficients(y)
+    xq = np.sort(rng.uniform(1.0, 57.0, 400))
+    ref = CubicSpline(np.arange(1, 58), y, axis=-1, bc_type="natural")(xq)
+    assert np.max(np.abs(O.bspline_eval(c, xq) - ref)) < 1e-12
+    # a dropped boundary row or a wrong tap weight would fail this: the
+    # interpolant passes through the data and its second derivative vanishes
+    # at the ends
+    assert np.max(np.abs(O.bspline_eval(c, np.arange(1.0, 58.0)) - y)) < 1e-12
+    assert np.allclose(c[:, 0] - 2 * c[:, 1] + c[:, 2], 0.0, atol=1e-12)
+    assert np.allclose(c[:, -3] - 2 * c[:, -2] + c[:, -1], 0.0, atol=1e-12)
+
+
+def test_resample_time_lines_flat_ends_and_amplitude_scale():
+    """Linear data are reproduced exactly (Line ends), t outside t_in takes
+    the end value (extrapolate(..., Flat())) and the output is scaled by
+    step(t_out)/step(t_in) (src/resample.jl:77)."""
+    t0, dt, n = -0.3, 0.01, 80
+    tin = t0 + dt * np.arange(n)
+    lin = 2.0 - 5.0 * tin
+    out = O.resample_time(lin[None, :], t0, dt, 0.0, 0.0137, 40, scale_amplitude=False)[0]
+    tout = 0.0137 * np.arange(40)
+    expect = np.where(tout > tin[-1], lin[-1], 2.0 - 5.0 * tout)
+    assert np.max(np.abs(out - expect)) < 1e-12
+    before = O.resample_time(lin[None, :], t0, dt, -1.0, 0.1, 3, scale_amplitude=False)[0]
+    assert np.max(np.abs(before - lin[0])) < 1e-13
+    y = np.sin(tin * 7.0)[None, :]
+    a = O.resample_time(y, t0, dt, 0.0, 0.02, 20, scale_amplitude=True)
+    b = O.resample_time(y, t0, dt, 0.0, 0.02, 20, scale_amplitude=False)
+    assert np.allclose(a, b * 2.0, rtol=1e-15, atol=0)
+
+
+def test_gf_denoise_uses_the_signed_maximum():
+    """greens.jl:176-177: A .= ifelse.(abs.(A) .< 1e-4 .* maximum(A, dims=1),
+    0, A) -- the threshold is the signed maximum, so an all-negative trace
+    keeps every sample, and the comparison is strict."""
+    a = np.array([[1.0, 5e-5, -5e-5, 1e-4, -2.0, 3e-5]])
+    d = O.gf_denoise(a)
+    assert d.tolist() == [[1.0, 0.0, 0.0, 1e-4, -2.0, 0.0]]
+    neg = np.array([[-1.0, -1e-9, -3.0]])
+    assert np.array_equal(O.gf_denoise(neg), neg)
