@@ -276,6 +276,30 @@ def plain_solves_leg(n_solves, nt):
             "solves_per_s": round(n_solves / wall, 3), "solves_per_s_device": round(n_solves / dev, 3)}
 
 
+def gfs_leg(n_src=32, n=128, nt=1200, n_rec=100, nt_save=600):
+    """impulsive_gfs end to end (src/greens.jl:48-234): 2 n_src solves (a
+    pressure and a velocity monopole per outer point) on the 128^3 batched
+    workload's grid, scatterer and surfaces, advanced as one batch, then the
+    device post-processing of every run (self-record dropped, denoise, cubic
+    resample to nt_save samples).  Wall time of the whole call, set-up and
+    host transfers included."""
+    import adfdtd as A
+
+    dx = C_AIR / (2.5 * FC) / 10
+    xmax = (n - 1) * dx / 2
+    d0 = A.make_domain(1.0, xmax, xmax, xmax, n, n, n, C_AIR, RHO_AIR, 0.5)
+    d = A.make_domain((nt - 0.5) * d0.dt, xmax, xmax, xmax, n, n, n, C_AIR, RHO_AIR, 0.5)
+    cfg = {"greens": {"f_3db_hz": FC, "nt_save": nt_save}, "pml": {"n": 10, "rcoef": 1e-4, "fc": FC},
+           "surfaces": {"nPoints_outer": n_src, "radius_outer": 40 * d.dx, "nPoints_inner": n_rec,
+                        "radius_inner": 20 * d.dx, "sphere_radius": 16 * d.dx}}
+    t0 = time.perf_counter()
+    g = A.impulsive_gfs(d, cfg, scatterer="sphere", sources_per_batch=n_src)
+    wall = time.perf_counter() - t0
+    return {"solves": 2 * n_src, "grid": n, "steps": nt, "receivers": n_rec, "nt_save": nt_save,
+            "wall_s": round(wall, 3), "solves_per_s": round(2 * n_src / wall, 3),
+            "p_p_peak": float(np.abs(g["p_p"]).max())}
+
+
 def kh_leg(which="p_v"):
     """Post-hoc KH of the 256^3 workload's surfaces: the 2000-point KH sphere
     (r = 40 cells) extrapolated to the 500 far-field points (r = 80 cells)
@@ -462,6 +486,10 @@ def main():
         res["batched_uneven"] = {"workload": f"48 solves, 16 each at 96^3/128^3/160^3 (radii scaled by n/128), "
                                              f"{ns} steps", **un}
     if not args.skip_batch:
+        res["impulsive_gfs"] = {"workload": "impulsive_gfs end to end: 32 outer points x (p, v) sources = 64 solves, "
+                                            "128^3 FP32, sphere r=16 cells, 100 inner receivers, 1200 steps, "
+                                            "resampled to 600 samples", **gfs_leg()}
+        log("impulsive_gfs leg done")
         res["kh_post_hoc"] = {"workload": "analytic KH p_v kernel, 2000-point KH sphere -> 500 far-field points, "
                                           "2000 steps, FP32 kernel samples and inputs, double sums", **kh_leg()}
         log("post-hoc KH leg done")
