@@ -486,11 +486,9 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
 enum { TILE_INT = 0, TILE_SY = 1, TILE_SZ = 2, TILE_SYZ = 3, TILE_GEN = 4 };
 enum { LIST_INT = 0, LIST_SIDE = 1, LIST_ALL = 2, TILE_LISTS = 3 };
 
-// One tile of a solve.  PF: the pipelined routine for interior and
-// single-slab tiles (tiles in both a y and a z slab, a few per cent of the
-// cells, take fused_tile: their pipelined form needs more registers than
-// the rest); otherwise the plane-at-a-time routines (fused_tile for all
-// side tiles).
+// One tile of a solve.  PF: the pipelined routine for every regular tile;
+// otherwise the plane-at-a-time routines (fused_tile for all side tiles).
+// General tiles always take fused_tile.
 template <typename T, bool EXACT, int CPL, bool PF>
 __device__ __forceinline__ void sweep_tile(const Fused<T>& F, uint2 e) {
     const int j0 = (int)(e.x & 0xffffu), k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0xffffu);
@@ -504,6 +502,8 @@ __device__ __forceinline__ void sweep_tile(const Fused<T>& F, uint2 e) {
         sweep_tile_pf<T, EXACT, CPL, TS, true, false>(F, j0, k0, k1, masked);
     } else if (PF && kind == TILE_SZ) {
         sweep_tile_pf<T, EXACT, CPL, TS, false, true>(F, j0, k0, k1, masked);
+    } else if (PF && kind == TILE_SYZ) {
+        sweep_tile_pf<T, EXACT, CPL, TS, true, true>(F, j0, k0, k1, masked);
     } else {
         fused_tile<T, EXACT, CPL, TS, true>(F, j0, k0, k1, masked);
     }
