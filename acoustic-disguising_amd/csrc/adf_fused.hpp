@@ -29,9 +29,11 @@ namespace adf {
 
 // Rows per wave of the fused sweep's tiles: interior tiles (SWEEP_TY, also
 // the row granularity of the tile split) and side tiles (CPML-slab and
-// boundary tiles, whose routines need more registers per row).
+// boundary tiles), whose routines need more registers per row: one row
+// at 4 cells per lane, where two rows would not fit the register budget of
+// the interior routine, else two.
 constexpr int SWEEP_TY = 2;
-constexpr int SWEEP_TY_SIDE = 1;
+__host__ __device__ constexpr int sweep_ty_side(int cpl) { return cpl >= 4 ? 1 : 2; }
 
 // Kernel arguments of one solve, kept small so the fused kernel does not
 // run out of scalar registers: one base pointer per buffer, offsets derived

@@ -723,9 +723,10 @@ static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, int W, uin
     // (k0, t, entry) in (z-block, y-tile) order, so neighbouring entries
     // share halo rows
     // Rows are classified in tiles of ty rows; side tiles are then cut into
-    // tiles of SWEEP_TY_SIDE rows (a regular tile's row subsets are regular,
+    // tiles of sweep_ty_side rows (a regular tile's row subsets are regular,
     // and the slab routines are exact for rows outside the slab).
     std::vector<std::tuple<int, int, uint2>> lists[TILE_LISTS];
+    const int tys = std::min(ty, sweep_ty_side(fused_cpl(s)));
     for (int t = 0; t < tiles_y; ++t) {
         const bool yg = rgen(t), ys = !yg && rslab(t);
         const int j0 = t * ty, j1 = std::min(j0 + ty, ny);
@@ -739,8 +740,8 @@ static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, int W, uin
                 lists[LIST_INT].emplace_back(z.first, j0, tile(j0, ty, z.first, z.second, kind));
                 continue;
             }
-            for (int j = j0; j < j1; j += SWEEP_TY_SIDE)
-                lists[LIST_SIDE].emplace_back(z.first, j, tile(j, SWEEP_TY_SIDE, z.first, z.second, kind));
+            for (int j = j0; j < j1; j += tys)
+                lists[LIST_SIDE].emplace_back(z.first, j, tile(j, tys, z.first, z.second, kind));
         }
     }
     auto order = [](const std::tuple<int, int, uint2>& a, const std::tuple<int, int, uint2>& b) {
