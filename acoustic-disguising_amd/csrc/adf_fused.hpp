@@ -129,6 +129,17 @@ __device__ __forceinline__ void bld(Rsrc r, int voff, int soff, T (&o)[N]) {
     }
 }
 
+// A 128-bit buffer store followed by eight wait states during which its
+// data registers are held (the asm reads them, so nothing can overwrite
+// them before it).  Without this, 256^3 float32 sweeps stored, in some
+// lanes, a value that a later VALU or LDS instruction wrote into the first
+// data register of such a store (results varying from run to run); the
+// compiler inserts no wait state after stores with an SGPR offset.
+__device__ __forceinline__ void store_b128(u32x4 u, Rsrc r, int voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b128(u, r, voff, soff, 0);
+    asm volatile("s_nop 7" ::"v"(u));
+}
+
 template <typename T, int N>
 __device__ __forceinline__ void bst(Rsrc r, int voff, int soff, const T (&o)[N]) {
     constexpr int B = (int)sizeof(T) * N;
@@ -143,14 +154,14 @@ __device__ __forceinline__ void bst(Rsrc r, int voff, int soff, const T (&o)[N])
     } else if constexpr (B == 16) {
         u32x4 u;
         __builtin_memcpy(&u, &o[0], 16);
-        __builtin_amdgcn_raw_buffer_store_b128(u, r, voff, soff, 0);
+        store_b128(u, r, voff, soff);
     } else {
         static_assert(B == 32, "row chunk of 4, 8, 16 or 32 bytes");
         u32x4 u0, u1;
         __builtin_memcpy(&u0, &o[0], 16);
         __builtin_memcpy(&u1, &o[N / 2], 16);
-        __builtin_amdgcn_raw_buffer_store_b128(u0, r, voff, soff, 0);
-        __builtin_amdgcn_raw_buffer_store_b128(u1, r, voff + 16, soff, 0);
+        store_b128(u0, r, voff, soff);
+        store_b128(u1, r, voff + 16, soff);
     }
 }
 

@@ -384,12 +384,24 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
                 const T vxn = q + 1 < CPL ? L.vx0[r][q + 1] : vxl;
                 A dvx = M_::diff(vxn, L.vx0[r][q], F.dx, F.rdx);
                 if (cpP) dvx = M_::psi(L.mPx[r][q], XP.b[q], XP.a[q], dvx);
+                // P at i = nx-1 is never updated (forward.jl:67-68): that
+                // cell keeps P and its y/z slab memory (the x slab excludes
+                // it through its range)
+                const bool upd = i0 + q < nx - 1;
                 A dvy = M_::diff(L.vy0[r + 1][q], L.vy0[r][q], F.dy, F.rdy);
-                if constexpr (SY) dvy = M_::psi(L.mPy[r][q], byP[r], ayP[r], dvy);
+                if constexpr (SY) {
+                    const T m0 = L.mPy[r][q];
+                    dvy = M_::psi(L.mPy[r][q], byP[r], ayP[r], dvy);
+                    if (!upd) L.mPy[r][q] = m0;
+                }
                 A dvz = M_::diff(L.vzn[r][q], vzc[r][q], F.dz, F.rdz);
-                if constexpr (SZ) dvz = M_::psi(L.mPz[r][q], L.bzP, L.azP, dvz);
+                if constexpr (SZ) {
+                    const T m0 = L.mPz[r][q];
+                    dvz = M_::psi(L.mPz[r][q], L.bzP, L.azP, dvz);
+                    if (!upd) L.mPz[r][q] = m0;
+                }
                 const T out = M_::upd(L.p0[r][q], F.fq, F.fqf, (mp >> q) & 1u, (dvx + dvy) + dvz);
-                p1[r][q] = (i0 + q < nx - 1) ? out : L.p0[r][q];
+                p1[r][q] = upd ? out : L.p0[r][q];
             }
         }
     };

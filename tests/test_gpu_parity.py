@@ -88,6 +88,20 @@ def test_cpml_guards(npml, shape, path):
     _compare(d, txs, cpml, mask, "f64", path=path)
 
 
+@pytest.mark.parametrize("kind", ["f64", "ref", "fast"])
+@pytest.mark.parametrize("shape,size", [("sphere", 5), ("cube", 20)])
+def test_random_field_every_cell_active(kind, shape, size, path):
+    # a random initial field makes every cell, every CPML memory entry and
+    # every boundary row/plane active from the first step; the cube of edge
+    # 20 cells reaches into the CPML slabs (masked cells inside the slabs)
+    d, txs, cpml, mask = build(n=(40, 29, 27), nt=12, npml=5, shape=shape, shape_params=[size * 0.00686],
+                               n_rec=10, rec_radius_cells=6)
+    f0 = np.random.default_rng(7).standard_normal((4, d.nz, d.ny, d.nx))
+    errs = _compare(d, txs, cpml, mask, kind, field=f0 if kind == "f64" else f0.astype(np.float32), path=path)
+    if kind == "f64":
+        assert all(v == 0.0 for v in errs.values()), errs
+
+
 def test_overlapping_injectors_keep_sequential_order(path):
     # several transceivers inject into shared cells: the result depends on the
     # order of the transceiver loop (derivatives.jl:30-60); levels keep it.
