@@ -26,7 +26,8 @@ _lock = threading.Lock()
 # block_z: planes per interior tile; block_z_gen: planes per side tile (CPML
 # slabs and boundary); pipeline: the prefetching tile routines; launch: one
 # launch over all tiles, or side and interior tiles as two launches on two
-# streams at the same time (single solves; ADF_SWEEP_*)
+# streams at the same time (single solves; ADF_SWEEP_*); store_nt: the
+# non-temporal store hint, tried on the best tiling of a single solve
 SOLVER_CANDIDATES = [dict(tile_y=0, block_z=kb, block_z_gen=kbg, waves=1, pipeline=pf, launch=ln)
                      for kb, kbg, pf, ln in itertools.product((4, 8, 16, 32), (2, 4), (1, 0), (0, 1))]
 BATCH_CANDIDATES = [dict(tile_y=0, block_z=kb, block_z_gen=kbg, waves=1, persistent=pers, pipeline=pf)
@@ -73,7 +74,8 @@ def _scratch(s: Solver) -> Solver:
 
 def _apply(s: Solver, shape: dict) -> None:
     s.set_path("auto", shape["tile_y"], shape["block_z"], shape["waves"], shape["block_z_gen"],
-               pipeline=shape.get("pipeline", 1), launch=shape.get("launch", L.ADF_SWEEP_ONE_LAUNCH))
+               pipeline=shape.get("pipeline", 1), launch=shape.get("launch", L.ADF_SWEEP_ONE_LAUNCH),
+               store_nt=shape.get("store_nt", 0))
 
 
 def tune_solver(s: Solver, steps: int = 20, candidates: Optional[Sequence[dict]] = None,
@@ -102,6 +104,18 @@ def tune_solver(s: Solver, steps: int = 20, candidates: Optional[Sequence[dict]]
         ms, _ = x.sweep_stats()
         if ms < best_ms:
             best, best_ms = dict(cand), ms
+    if best is not None and best.get("pipeline", 1) and candidates is None:
+        # the store hint on the best tiling, both timed again over more steps
+        timed = {}
+        m = max(1, min(2 * n, s.dom.nt - 2))
+        for nt_ in (0, 1):
+            _apply(x, dict(best, store_nt=nt_))
+            x.reset()
+            x.run(1, 2)
+            x.run(3, 2 + m)
+            timed[nt_], _ = x.sweep_stats()
+        best["store_nt"] = int(timed[1] < timed[0])
+        best_ms = timed[best["store_nt"]]
     del x
     if best is None:
         best = dict(SOLVER_CANDIDATES[0])

@@ -135,33 +135,38 @@ __device__ __forceinline__ void bld(Rsrc r, int voff, int soff, T (&o)[N]) {
 // lanes, a value that a later VALU or LDS instruction wrote into the first
 // data register of such a store (results varying from run to run); the
 // compiler inserts no wait state after stores with an SGPR offset.
+template <int POL>
 __device__ __forceinline__ void store_b128(u32x4 u, Rsrc r, int voff, int soff) {
-    __builtin_amdgcn_raw_buffer_store_b128(u, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(u, r, voff, soff, POL);
     asm volatile("s_nop 7" ::"v"(u));
 }
 
-template <typename T, int N>
+// Cache policy of a store: 0 ordinary, ST_NT the non-temporal hint (the
+// instruction's nt bit).
+constexpr int ST_NT = 2;
+
+template <typename T, int N, int POL = 0>
 __device__ __forceinline__ void bst(Rsrc r, int voff, int soff, const T (&o)[N]) {
     constexpr int B = (int)sizeof(T) * N;
     if constexpr (B == 4) {
         uint32_t u;
         __builtin_memcpy(&u, &o[0], 4);
-        __builtin_amdgcn_raw_buffer_store_b32(u, r, voff, soff, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(u, r, voff, soff, POL);
     } else if constexpr (B == 8) {
         u32x2 u;
         __builtin_memcpy(&u, &o[0], 8);
-        __builtin_amdgcn_raw_buffer_store_b64(u, r, voff, soff, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(u, r, voff, soff, POL);
     } else if constexpr (B == 16) {
         u32x4 u;
         __builtin_memcpy(&u, &o[0], 16);
-        store_b128(u, r, voff, soff);
+        store_b128<POL>(u, r, voff, soff);
     } else {
         static_assert(B == 32, "row chunk of 4, 8, 16 or 32 bytes");
         u32x4 u0, u1;
         __builtin_memcpy(&u0, &o[0], 16);
         __builtin_memcpy(&u1, &o[N / 2], 16);
-        store_b128(u0, r, voff, soff);
-        store_b128(u1, r, voff + 16, soff);
+        store_b128<POL>(u0, r, voff, soff);
+        store_b128<POL>(u1, r, voff + 16, soff);
     }
 }
 
@@ -344,7 +349,7 @@ struct FA {
 // planes, so only the x direction keeps special cases.  Loads use clamped
 // indices (always in bounds) instead of guards; values of rows, lanes and
 // planes outside the grid are computed but never stored.
-template <typename T, bool EXACT, int CPL, int TY, bool GEN>
+template <typename T, bool EXACT, int CPL, int TY, bool GEN, int POL = 0>
 __device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, int k1, bool masked) {
     typedef FA<T, EXACT> M_;
     typedef typename M_::A A;
@@ -525,8 +530,8 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, in
                 }
                 T ayV = (T)0, byV = (T)0;
                 if constexpr (GEN) {
-                    if (syP[r] && !rj) bst<T, CPL>(rS1, vs, (F.mv_off(1) + nx * (lyP[r] + p * (k + nz * (syP[r] - 1)))) * ES, mPy[r]);
-                    if (szP && !rj) bst<T, CPL>(rS1, vs, (F.mv_off(2) + nx * (j + ny * (lzP + p * (szP - 1)))) * ES, mPz[r]);
+                    if (syP[r] && !rj) bst<T, CPL, POL>(rS1, vs, (F.mv_off(1) + nx * (lyP[r] + p * (k + nz * (syP[r] - 1)))) * ES, mPy[r]);
+                    if (szP && !rj) bst<T, CPL, POL>(rS1, vs, (F.mv_off(2) + nx * (j + ny * (lzP + p * (szP - 1)))) * ES, mPz[r]);
                     if (syV[r]) { ayV = F.cah(lyV[r] + qv * (syV[r] - 1)); byV = F.cbh(lyV[r] + qv * (syV[r] - 1)); }
                 }
                 T vx1[CPL], vy1[CPL], vz1[CPL];
@@ -557,22 +562,22 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, in
                         vz1[q] = M_::upd(vzc[r][q], F.fv, F.fvf, (mvz_[r] >> q) & 1u, dp);
                     }
                 }
-                bst<T, CPL>(rF1, vs, rb, p1[r]);
-                bst<T, CPL>(rF1, vs, chrow(1, rb), vx1);
-                if (!GEN || j >= 1 || !F.fuse_mirror) bst<T, CPL>(rF1, vs, chrow(2, rb), vy1);
-                if (!GEN || k >= 1 || !F.fuse_mirror) bst<T, CPL>(rF1, vs, chrow(3, rb), vz1);
+                bst<T, CPL, POL>(rF1, vs, rb, p1[r]);
+                bst<T, CPL, POL>(rF1, vs, chrow(1, rb), vx1);
+                if (!GEN || j >= 1 || !F.fuse_mirror) bst<T, CPL, POL>(rF1, vs, chrow(2, rb), vy1);
+                if (!GEN || k >= 1 || !F.fuse_mirror) bst<T, CPL, POL>(rF1, vs, chrow(3, rb), vz1);
                 if constexpr (GEN) {
                     if (F.fuse_mirror && (j == 1 || k == 1)) {
                         T neg[CPL];
                         if (j == 1) {  // forward.jl:135
 #pragma unroll
                             for (int q = 0; q < CPL; ++q) neg[q] = -vy1[q];
-                            bst<T, CPL>(rF1, vs, chrow(2, (int)((uint32_t)rb - (uint32_t)(nx * ES))), neg);
+                            bst<T, CPL, POL>(rF1, vs, chrow(2, (int)((uint32_t)rb - (uint32_t)(nx * ES))), neg);
                         }
                         if (k == 1) {  // forward.jl:136
 #pragma unroll
                             for (int q = 0; q < CPL; ++q) neg[q] = -vz1[q];
-                            bst<T, CPL>(rF1, vs, chrow(3, (int)((uint32_t)rb - (uint32_t)(sxy * ES))), neg);
+                            bst<T, CPL, POL>(rF1, vs, chrow(3, (int)((uint32_t)rb - (uint32_t)(sxy * ES))), neg);
                         }
                     }
                 }
@@ -582,8 +587,8 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, in
                         if (sxV[q]) psi1[F.mp_off(0) + lxV[q] + qv * (j + ny * (k + nz * (sxV[q] - 1)))] = mVx[r][q];
                 }
                 if constexpr (GEN) {
-                    if (syV[r]) bst<T, CPL>(rS1, vs, (F.mp_off(1) + nx * (lyV[r] + qv * (k + nz * (syV[r] - 1)))) * ES, mVy[r]);
-                    if (szV) bst<T, CPL>(rS1, vs, (F.mp_off(2) + nx * (j + ny * (lzV + qv * (szV - 1)))) * ES, mVz[r]);
+                    if (syV[r]) bst<T, CPL, POL>(rS1, vs, (F.mp_off(1) + nx * (lyV[r] + qv * (k + nz * (syV[r] - 1)))) * ES, mVy[r]);
+                    if (szV) bst<T, CPL, POL>(rS1, vs, (F.mp_off(2) + nx * (j + ny * (lzV + qv * (szV - 1)))) * ES, mVz[r]);
                 }
             }
         }

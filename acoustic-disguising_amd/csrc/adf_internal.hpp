@@ -95,6 +95,7 @@ struct adf_solver {
     int waves = 0;     // waves per workgroup of the fused kernel (0 auto)
     int block_z_gen = 0;  // planes per general tile (0 auto)
     int pipeline = 1;     // pipelined (prefetching) tile routines
+    int store_nt = 0;     // non-temporal stores in the pipelined routines
     int launch = 0;       // ADF_SWEEP_* (one launch, two streams, two serial)
 
     // GF extrapolation map
@@ -113,7 +114,8 @@ struct adf_solver {
     int64_t next_it = 1;          // next step a batch run takes (1-based)
     adf_batch* batch = nullptr;   // batch this solver belongs to, if any
     // optional per-launch timing of the field-update kernel
-    bool time_sweep = false;
+    int time_sweep = 0;           // every time_sweep-th step (0: off)
+    int64_t sweep_seq = 0;        // steps of the current run so far
     std::vector<hipEvent_t> sweep_ev;  // pairs (start, stop)
     int sweep_used = 0;
     double sweep_avg_ms = 0.0;

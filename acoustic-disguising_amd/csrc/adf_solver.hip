@@ -449,7 +449,8 @@ int solver_set_option(adf_solver* s, int option, int64_t value) {
         s->tile_y = (int)value;
         touch(s);
     } else if (option == ADF_OPT_TIME_SWEEP) {
-        s->time_sweep = value != 0;
+        if (value < 0 || value > (1 << 30)) return fail(ADF_EINVAL, "ADF_OPT_TIME_SWEEP must be >= 0");
+        s->time_sweep = (int)value;
     } else if (option == ADF_OPT_BLOCK_Z) {
         if (value < 0 || value > (1 << 20)) return fail(ADF_EINVAL, "ADF_OPT_BLOCK_Z out of range");
         s->block_z = (int)value;
@@ -462,6 +463,9 @@ int solver_set_option(adf_solver* s, int option, int64_t value) {
         if (value != 0 && value != 1 && value != 2 && value != 4) return fail(ADF_EINVAL, "ADF_OPT_WAVES must be 0, 1, 2 or 4");
         s->waves = (int)value;
         touch(s);
+    } else if (option == ADF_OPT_STORE_NT) {
+        if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_STORE_NT must be 0 or 1");
+        s->store_nt = (int)value;
     } else if (option == ADF_OPT_PIPELINE) {
         if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_PIPELINE must be 0 or 1");
         s->pipeline = (int)value;
@@ -795,9 +799,10 @@ static int prepare_tiles(adf_solver* s, int ty, int kb, int kbg) {
     return ADF_OK;
 }
 
-// Start/stop events around one field-update launch (ADF_OPT_TIME_SWEEP).
-static void sweep_mark(adf_solver* s, hipStream_t st) {
-    if (!s->time_sweep) return;
+// Start/stop events around one field-update launch (ADF_OPT_TIME_SWEEP) of
+// a sampled step.
+static void sweep_mark(adf_solver* s, hipStream_t st, bool sampled) {
+    if (!sampled) return;
     if ((int)s->sweep_ev.size() <= s->sweep_used) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return;
@@ -811,11 +816,13 @@ template <typename T, bool EXACT>
 static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int ty) {
     const int it0 = (int)(it - 1);
     Txs<T> X = make_txs<T>(s);
+    const bool mark = s->time_sweep > 0 && s->sweep_seq % s->time_sweep == 0;
+    ++s->sweep_seq;
     if (fused) {
         Fused<T> F = make_fused<T>(s, ty);
-        sweep_mark(s, st);
+        sweep_mark(s, st, mark);
         launch_fused<T, EXACT>(s, F, fused_cpl(s), st);
-        sweep_mark(s, st);
+        sweep_mark(s, st, mark);
         s->cur ^= 1;  // the new buffers become current
         Solve<T> S = make_solve<T>(s);
         if (s->ntx) {
@@ -834,13 +841,13 @@ static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int
         const int kz = s->block_z ? s->block_z : KZ;
         const dim3 gp((S.nx - 1 + 63) / 64, (S.ny + 3) / 4, (S.nz + kz - 1) / kz);
         const dim3 gv((S.nx + 63) / 64, (S.ny + 3) / 4, (S.nz + kz - 1) / kz);
-        sweep_mark(s, st);
+        sweep_mark(s, st, mark);
         hipLaunchKernelGGL((k_update_p<T, EXACT>), gp, blk, 0, st, S, kz);
-        sweep_mark(s, st);
+        sweep_mark(s, st, mark);
         if (s->ntx) launch_tx_p<T>(s, S, X, it0, st);
-        sweep_mark(s, st);
+        sweep_mark(s, st, mark);
         hipLaunchKernelGGL((k_update_v<T, EXACT>), gv, blk, 0, st, S, kz);
-        sweep_mark(s, st);
+        sweep_mark(s, st, mark);
         launch_after_v<T>(s, S, X, it, st);
     }
 }
@@ -858,6 +865,7 @@ static int run_impl(adf_solver* s, int64_t start, int64_t end) {
     s->sweep_used = 0;
     s->sweep_avg_ms = 0.0;
     s->sweep_n = 0;
+    s->sweep_seq = 0;
     HIPCHK(hipEventRecord(s->ev0, st));
     for (int64_t it = start; it <= end; ++it) {
         step_impl<T, EXACT>(s, it, st, fused, ty);

@@ -255,7 +255,7 @@ struct SPlane : IPlane<T, CPL, TY> {
 // zero-reading null resource) outside the slab, which leaves the derivative
 // unchanged, as the reference's slab test does (updates.jl:21-103,192-275,
 // 320-378,380-501).  The x slab is handled the same way per lane.
-template <typename T, bool EXACT, int CPL, int TY, bool SY, bool SZ>
+template <typename T, bool EXACT, int CPL, int TY, bool SY, bool SZ, int POL = 0>
 __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0, int k1, bool masked) {
     typedef FA<T, EXACT> M_;
     typedef typename M_::A A;
@@ -440,26 +440,26 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
                 if constexpr (SZ) dpz = M_::psi(L.mVz[r][q], L.bzV, L.azV, dpz);
                 vz1[q] = M_::upd(vzc[r][q], F.fv, F.fvf, (mvz >> q) & 1u, dpz);
             }
-            bst<T, CPL>(rF1, vs, rb, p1[r]);
-            bst<T, CPL>(rF1, vs, chrow(1, rb), vx1);
-            bst<T, CPL>(rF1, vs, chrow(2, rb), vy1);
-            bst<T, CPL>(rF1, vs, chrow(3, rb), vz1);
+            bst<T, CPL, POL>(rF1, vs, rb, p1[r]);
+            bst<T, CPL, POL>(rF1, vs, chrow(1, rb), vx1);
+            bst<T, CPL, POL>(rF1, vs, chrow(2, rb), vy1);
+            bst<T, CPL, POL>(rF1, vs, chrow(3, rb), vz1);
             const int soP = p * (j + ny * k) * ES, soV = qv * (j + ny * k) * ES;
-            bst<T, CPL>(rS1, sPv, soP, L.mPx[r]);
-            bst<T, CPL>(rS1, sVv, soV, L.mVx[r]);
+            bst<T, CPL, POL>(rS1, sPv, soP, L.mPx[r]);
+            bst<T, CPL, POL>(rS1, sVv, soV, L.mVx[r]);
 #pragma unroll
             for (int q = 0; q < CPL; ++q) {
                 const T a1[1] = {L.mPx[r][q]}, b1[1] = {L.mVx[r][q]};
-                bst<T, 1>(rS1, sPq[q], soP, a1);
-                bst<T, 1>(rS1, sVq[q], soV, b1);
+                bst<T, 1, POL>(rS1, sPq[q], soP, a1);
+                bst<T, 1, POL>(rS1, sVq[q], soV, b1);
             }
             if constexpr (SY) {
-                bst<T, CPL>(rsrc_of(F.psi1, yinP[r] ? psib : 0u), vs, yoP[r] + k * ypP, L.mPy[r]);
-                bst<T, CPL>(rsrc_of(F.psi1, yinV[r] ? psib : 0u), vs, yoV[r] + k * ypV, L.mVy[r]);
+                bst<T, CPL, POL>(rsrc_of(F.psi1, yinP[r] ? psib : 0u), vs, yoP[r] + k * ypP, L.mPy[r]);
+                bst<T, CPL, POL>(rsrc_of(F.psi1, yinV[r] ? psib : 0u), vs, yoV[r] + k * ypV, L.mVy[r]);
             }
             if constexpr (SZ) {
-                bst<T, CPL>(rsrc_of(F.psi1, L.zinP ? psib : 0u), vs, L.zoP + r * nx * ES, L.mPz[r]);
-                bst<T, CPL>(rsrc_of(F.psi1, L.zinV ? psib : 0u), vs, L.zoV + r * nx * ES, L.mVz[r]);
+                bst<T, CPL, POL>(rsrc_of(F.psi1, L.zinP ? psib : 0u), vs, L.zoP + r * nx * ES, L.mPz[r]);
+                bst<T, CPL, POL>(rsrc_of(F.psi1, L.zinV ? psib : 0u), vs, L.zoV + r * nx * ES, L.mVz[r]);
             }
         }
         carry(L, p1);
@@ -501,35 +501,36 @@ enum { LIST_INT = 0, LIST_SIDE = 1, LIST_ALL = 2, TILE_LISTS = 3 };
 // One tile of a solve.  PF: the pipelined routine for every regular tile;
 // otherwise the plane-at-a-time routines (fused_tile for all side tiles).
 // General tiles always take fused_tile.
-template <typename T, bool EXACT, int CPL, bool PF>
+template <typename T, bool EXACT, int CPL, bool PF, int POL = 0>
 __device__ __forceinline__ void sweep_tile(const Fused<T>& F, uint2 e) {
     const int j0 = (int)(e.x & 0xffffu), k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0xffffu);
     const bool masked = (e.x >> 31) != 0;
     const int kind = (int)((e.x >> 28) & 7u);
     constexpr int TI = SWEEP_TY, TS = sweep_ty_side(CPL);
     if (kind == TILE_INT) {
-        if constexpr (PF) sweep_tile_pf<T, EXACT, CPL, TI, false, false>(F, j0, k0, k1, masked);
+        if constexpr (PF) sweep_tile_pf<T, EXACT, CPL, TI, false, false, POL>(F, j0, k0, k1, masked);
         else sweep_int_tile<T, EXACT, CPL, TI>(F, j0, k0, k1, masked);
     } else if (PF && kind == TILE_SY) {
-        sweep_tile_pf<T, EXACT, CPL, TS, true, false>(F, j0, k0, k1, masked);
+        sweep_tile_pf<T, EXACT, CPL, TS, true, false, POL>(F, j0, k0, k1, masked);
     } else if (PF && kind == TILE_SZ) {
-        sweep_tile_pf<T, EXACT, CPL, TS, false, true>(F, j0, k0, k1, masked);
+        sweep_tile_pf<T, EXACT, CPL, TS, false, true, POL>(F, j0, k0, k1, masked);
     } else if (PF && kind == TILE_SYZ) {
-        sweep_tile_pf<T, EXACT, CPL, TS, true, true>(F, j0, k0, k1, masked);
+        sweep_tile_pf<T, EXACT, CPL, TS, true, true, POL>(F, j0, k0, k1, masked);
     } else {
-        fused_tile<T, EXACT, CPL, TS, true>(F, j0, k0, k1, masked);
+        fused_tile<T, EXACT, CPL, TS, true, POL>(F, j0, k0, k1, masked);
     }
 }
 
 // One solve's tiles of one list (entries as xcd_tile describes); W waves
-// per workgroup take W consecutive tiles.
-template <typename T, bool EXACT, int CPL, bool PF>
+// per workgroup take W consecutive tiles.  POL: cache policy of the
+// stores (0 or ST_NT).
+template <typename T, bool EXACT, int CPL, bool PF, int POL>
 __global__ __launch_bounds__(256) void k_sweep(Fused<T> F, const uint2* __restrict__ tiles, int ntiles) {
     const int W = (int)(blockDim.x >> 6);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x) * W + w;
     if (t >= ntiles) return;
-    sweep_tile<T, EXACT, CPL, PF>(F, tiles[t]);
+    sweep_tile<T, EXACT, CPL, PF, POL>(F, tiles[t]);
 }
 
 }  // namespace adf

@@ -14,10 +14,13 @@ static void launch_list(adf_solver* s, const Fused<T>& F, hipStream_t st) {
     if (!n) return;
     const int W = s->waves ? s->waves : 1;
     const dim3 g((unsigned)((n + W - 1) / W));
-    if (s->pipeline)
-        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true>), g, dim3(64 * W), 0, st, F, (const uint2*)s->tiles[LIST], n);
+    const uint2* tl = (const uint2*)s->tiles[LIST];
+    if (s->pipeline && s->store_nt)
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, ST_NT>), g, dim3(64 * W), 0, st, F, tl, n);
+    else if (s->pipeline)
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, 0>), g, dim3(64 * W), 0, st, F, tl, n);
     else
-        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, false>), g, dim3(64 * W), 0, st, F, (const uint2*)s->tiles[LIST], n);
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, false, 0>), g, dim3(64 * W), 0, st, F, tl, n);
 }
 
 // One launch over all tiles, or side and interior tiles as two launches

@@ -185,10 +185,14 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
 #define ADF_OPT_PATH 1
 #define ADF_OPT_TILE_Y 2
 #define ADF_OPT_BLOCK_Z 3
-/*   ADF_OPT_TIME_SWEEP  1: record HIP events around every launch of the
- *                        field-update kernel (fused sweep, or the P and V
- *                        kernels of the two-pass path) for
- *                        adf_solver_sweep_stats; 0: off (default). */
+/*   ADF_OPT_TIME_SWEEP  v >= 1: record HIP events around the field-update
+ *                        launches (fused sweep, or the P and V kernels of
+ *                        the two-pass path) of every v-th step of a run
+ *                        (the first step of the run included) for
+ *                        adf_solver_sweep_stats; 0: off (default).  Each
+ *                        recorded event pair adds device time to its step
+ *                        (about 18 us at 256^3), so sampled timing (v of
+ *                        16 or more) keeps the run's own step time. */
 #define ADF_OPT_TIME_SWEEP 4
 /*   ADF_OPT_WAVES    waves per workgroup of the fused kernel: 0 auto, 1, 2, 4
  *                    (a workgroup takes that many y-neighbouring tiles) */
@@ -212,6 +216,12 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
 #define ADF_SWEEP_ONE_LAUNCH 0
 #define ADF_SWEEP_TWO_STREAMS 1
 #define ADF_SWEEP_TWO_LAUNCHES 2
+/*   ADF_OPT_STORE_NT     1: the fused kernel's pipelined routines store
+ *                        with the non-temporal hint (faster where the two
+ *                        field buffers exceed the GPU's last-level cache,
+ *                        e.g. 256^3; slower where they fit); 0 (default):
+ *                        ordinary stores.  Results are identical. */
+#define ADF_OPT_STORE_NT 9
 int adf_solver_set_option(adf_solver* s, int option, int64_t value);
 
 /* Mean device time (ms) of one field-update launch and the number of

@@ -32,25 +32,37 @@ def _state(s, cp_dtype, d, npml):
     return s.get_field(), [s.get_trace(w) for w in range(10)], list(cp.p) + list(cp.v)
 
 
-def _compare_with_oracle(d, txs, npml, mask_shape, mask_params, field, steps):
+def _compare_with_oracle(d, txs, npml, mask_shape, mask_params, field, steps, store_nt=False):
+    """store_nt: also run the float32 solve with non-temporal stores, which
+    must give the same bits."""
     mask = A.build_update_mask(d, mask_shape, mask_params)
     cpml = A.make_cpml(d, npml, 1e-4, FC)
     ref = oracle_solver(O, d, txs, cpml, mask, "f64", field=field).run(1, steps)
     ref_psi = ref.mp + ref.mv
     out = {}
-    for dtype in (np.float64, np.float32):
+    runs = [(np.float64, 0), (np.float32, 0)] + ([(np.float32, 1)] if store_nt else [])
+    for dtype, nt in runs:
         s = A.Solver(d, A.dtype_code(dtype, "fast"), A.make_cpml(d, npml, 1e-4, FC, dtype))
         s.build_mask(mask_shape, mask_params)
         s.set_field(np.asarray(field, dtype))
         s.set_transceivers(A.transceivers_to_grid(d, txs, dtype))
+        s.set_path("auto", store_nt=nt)
         s.run(1, steps)
         f, tr, psi = _state(s, dtype, d, npml)
+        if nt:
+            a = out["f32_state"]
+            assert np.array_equal(f, a[0]) and all(np.array_equal(x, y) for x, y in zip(tr + psi, a[1] + a[2]))
+            del s
+            continue
+        if dtype == np.float32:
+            out["f32_state"] = (f, tr, psi)
         errs = {"field": rel_l2(f, ref.field),
                 "psi": max(rel_l2(a, b) for a, b in zip(psi, ref_psi)),
                 "p_rec": rel_l2(tr[0], ref.tx["rec"][0]), "vn_rec": rel_l2(tr[4], ref.tx["vn_rec"]),
                 "src": rel_l2(np.stack(tr[5:9]), ref.tx["src"])}
         out[dtype] = errs
         del s
+    out.pop("f32_state", None)
     print(out, flush=True)
     assert all(v == 0.0 for v in out[np.float64].values()), out[np.float64]
     assert all(v <= 1e-5 for v in out[np.float32].values()), out[np.float32]
@@ -70,7 +82,7 @@ def test_256_sphere_cpml16_2501_transceivers_full_size():
     txs += [A.make_transceiver(d, kh[i], np.r_[0.0, khn[i]], z) for i in range(2000)]
     txs += [A.make_transceiver(d, ff[i], np.r_[0.0, ffn[i]], z) for i in range(500)]
     field = np.random.default_rng(1).standard_normal((4, 256, 256, 256))
-    _compare_with_oracle(d, txs, 16, "sphere", [24 * d.dx], field, steps)
+    _compare_with_oracle(d, txs, 16, "sphere", [24 * d.dx], field, steps, store_nt=True)
 
 
 def test_192_hologram_cube_kh_injection_full_size():
