@@ -9,7 +9,7 @@ from .simulation import (CHANNELS, Cpml, Domain, Transceiver, TransceiverGrid, c
                          reset_cpml_memory, run_fdtd, t, transceivers_to_grid, trilinear_interpolation, vn_rec,
                          vn_src, x, xv, y, yv, z, zv)
 from .geometry import build_scatterer, build_update_mask, outside_predicate
-from .illumination import c_for_n, fibonacci_sphere, illumination_points, impulsive_wavelet, ricker_wavelet
+from .illumination import c_for_n, fibonacci_sphere, illumination_points, impulsive_wavelet, r2_sphere, ricker_wavelet
 from .extrapolation import PFromPin, PVFromPin, PVFromPV
 from .solver import Solver, dtype_code
 from .batch import Batch
@@ -17,5 +17,6 @@ from .server import Server
 from .autotune import tune_batch, tune_solver
 from .kh import kh_analytic_accumulate
 from .greens import impulsive_gfs, scatterer_params
+from .reverb import build_reverb_data, julia_step_range, resample_traces
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -25,6 +25,29 @@ def fibonacci_sphere(n: int, radius: float = 1.0):
     return pts, nrm
 
 
+def r2_sphere(n: int, radius: float = 1.0):
+    """src/illumination.jl:69-97 -> (points (n,3), normals (n,3)): Roberts'
+    R2 low-discrepancy sequence, u = frac(i/phi), v = frac(i/phi^2) with
+    phi the plastic number (the real root of x^3 = x + 1), mapped to the
+    sphere by the Lambert equal-area map.  Point i depends only on i."""
+    if n <= 0:
+        raise ValueError("r2_sphere requires n > 0")
+    phi = (9 + math.sqrt(69)) ** (1 / 3) / np.cbrt(18.0) + (9 - math.sqrt(69)) ** (1 / 3) / np.cbrt(18.0)
+    a1, a2 = 1.0 / phi, 1.0 / phi ** 2
+    pts = np.empty((n, 3))
+    nrm = np.empty((n, 3))
+    for i in range(1, n + 1):
+        u = math.fmod(i * a1, 1.0)
+        v = math.fmod(i * a2, 1.0)
+        ph = 2 * math.pi * u
+        zz = 1.0 - 2.0 * v
+        r_xy = math.sqrt(max(0.0, 1.0 - zz * zz))
+        nx, ny = r_xy * math.cos(ph), r_xy * math.sin(ph)
+        nrm[i - 1] = (nx, ny, zz)
+        pts[i - 1] = (radius * nx, radius * ny, radius * zz)
+    return pts, nrm
+
+
 def illumination_points(cfg: dict):
     """src/illumination.jl:119-130."""
     s = cfg["surfaces"]

@@ -1,6 +1,7 @@
-// impulsive_gfs post-processing of one finished run on the device
-// (include/adfdtd.h adf_solver_gf_resample; src/greens.jl:166-191):
+// Cubic resampling of recorded traces on the device:
 //
+// impulsive_gfs post-processing of one finished run (include/adfdtd.h
+// adf_solver_gf_resample; src/greens.jl:166-191):
 //   A      = receiver traces p and vn of transceivers first .. first+n_rec-1
 //            (greens.jl:171-174; first = 1 drops the source's self-record)
 //   A     := 0 where |A| < 1e-4 max_t A, the signed maximum of each trace
@@ -12,6 +13,10 @@
 //            Flat outside, sampled on t_gf = range(0, tmax; length =
 //            nt_gf), times step(t_gf) / step(t_shifted), stored as float32.
 //
+// the reverb stage's per-source resample (adf_resample_f32;
+// src/reverb.jl:424-452): the same interpolant through float32 traces the
+// host has already cast and scaled, without denoise or amplitude scaling.
+//
 // One thread per trace solves the spline system along time (Thomas
 // algorithm in double, the shared pivot sequence computed on the host);
 // one thread per output sample evaluates the four B-spline taps.  Along the
@@ -19,6 +24,7 @@
 // only at integer nodes, where the interpolant equals the data.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -37,25 +43,28 @@ struct DevBuf {
     }
 };
 
-// Denoise trace col and solve for its spline coefficients c_0 .. c_{n+1}
-// (rows (c_{i-1} + 4 c_i + c_{i+1}) / 6 = y_i, c_1 = y_1, c_n = y_n, zero
-// second difference at both ends).  cp: pivots 1/(4 - cp_{i-1}), cp_0 = 1/4.
+// Solve for the spline coefficients c_0 .. c_{n+1} of column col (rows
+// (c_{i-1} + 4 c_i + c_{i+1}) / 6 = y_i, c_1 = y_1, c_n = y_n, zero second
+// difference at both ends), after the denoise when asked.  Columns
+// 0 .. nsplit-1 are traces first .. of a0, the rest traces first .. of a1
+// (each n samples long).  cp: pivots 1/(4 - cp_{i-1}), cp_0 = 1/4.
 template <typename T>
-__global__ void k_gf_coef(const T* __restrict__ rec_p, const T* __restrict__ rec_vn, int nt, int first, int n_rec,
-                          const double* __restrict__ cp, double* __restrict__ coef) {
+__global__ void k_rs_coef(const T* __restrict__ a0, const T* __restrict__ a1, int64_t first, int nsplit, int n,
+                          int ncol, int denoise, const double* __restrict__ cp, double* __restrict__ coef) {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= 2 * n_rec) return;
-    const int which = col / n_rec, r = col - which * n_rec;
-    const T* a = (which ? rec_vn : rec_p) + (size_t)(first + r) * nt;
-    double* c = coef + (size_t)col * (nt + 2);
-    double mx = (double)a[0];
-    for (int k = 1; k < nt; ++k) mx = fmax(mx, (double)a[k]);
-    const double thr = 1e-4 * mx;
-    auto y = [&](int k) {  // denoised sample k (0-based)
+    if (col >= ncol) return;
+    const T* a = col < nsplit ? a0 + (size_t)(first + col) * n : a1 + (size_t)(first + col - nsplit) * n;
+    double* c = coef + (size_t)col * (n + 2);
+    double thr = -1.0;  // no sample is below it
+    if (denoise) {
+        double mx = (double)a[0];
+        for (int k = 1; k < n; ++k) mx = fmax(mx, (double)a[k]);
+        thr = 1e-4 * mx;
+    }
+    auto y = [&](int k) {  // (denoised) sample k (0-based)
         const double v = (double)a[k];
         return fabs(v) < thr ? 0.0 : v;
     };
-    const int n = nt;
     if (n == 1) {
         c[0] = c[1] = c[2] = y(0);
         return;
@@ -76,7 +85,7 @@ __global__ void k_gf_coef(const T* __restrict__ rec_p, const T* __restrict__ rec
 
 // out[col][k] = scale * s_col(x_k), x_k = 1 + (t_out0 + k dt_out - t_in0) / dt_in
 // clamped to [1, n].
-__global__ void k_gf_eval(const double* __restrict__ coef, int n, int ncol, double t_in0, double dt_in,
+__global__ void k_rs_eval(const double* __restrict__ coef, int n, int ncol, double t_in0, double dt_in,
                           double t_out0, double dt_out, int n_out, double scale, float* __restrict__ out) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (int64_t)ncol * n_out) return;
@@ -98,6 +107,31 @@ __global__ void k_gf_eval(const double* __restrict__ coef, int n, int ncol, doub
 
 }  // namespace
 
+// The spline resample of ncol columns (see k_rs_coef) on stream st into
+// host out (n_out samples per column, column after column).
+template <typename T>
+static int resample_columns(hipStream_t st, const T* a0, const T* a1, int64_t first, int64_t nsplit, int64_t n,
+                            int64_t ncol, int denoise, double t_in0, double dt_in, double t_out0, double dt_out,
+                            int64_t n_out, double scale, float* out) {
+    std::vector<double> cp((size_t)std::max<int64_t>(n - 2, 1));
+    cp[0] = 1.0 / 4.0;
+    for (size_t i = 1; i < cp.size(); ++i) cp[i] = 1.0 / (4.0 - cp[i - 1]);
+    DevBuf dcp, dcoef, dout;
+    HIPCHK(hipMalloc(&dcp.p, cp.size() * 8));
+    HIPCHK(hipMalloc(&dcoef.p, (size_t)ncol * (n + 2) * 8));
+    HIPCHK(hipMalloc(&dout.p, (size_t)ncol * n_out * 4));
+    HIPCHK(hipMemcpyAsync(dcp.p, cp.data(), cp.size() * 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL((k_rs_coef<T>), dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, st, a0, a1, first,
+                       (int)nsplit, (int)n, (int)ncol, denoise, (const double*)dcp.p, (double*)dcoef.p);
+    const int64_t tot = ncol * n_out;
+    hipLaunchKernelGGL(k_rs_eval, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, (const double*)dcoef.p,
+                       (int)n, (int)ncol, t_in0, dt_in, t_out0, dt_out, (int)n_out, scale, (float*)dout.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout.p, (size_t)tot * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return ADF_OK;
+}
+
 extern "C" int adf_solver_gf_resample(adf_solver* s, int64_t first, int64_t n_rec, double timeshift, int64_t nt_gf,
                                       float* out_p, float* out_vn) {
     if (!s) return fail(ADF_EINVAL, "adf_solver_gf_resample: null solver");
@@ -109,37 +143,42 @@ extern "C" int adf_solver_gf_resample(adf_solver* s, int64_t first, int64_t n_re
         return fail(ADF_EINVAL, "adf_solver_gf_resample: too many samples");
     HIPCHK(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
-    // pivots of the inner tridiagonal system (the same for every trace)
-    std::vector<double> cp((size_t)std::max<int64_t>(nt - 2, 1));
-    cp[0] = 1.0 / 4.0;
-    for (size_t i = 1; i < cp.size(); ++i) cp[i] = 1.0 / (4.0 - cp[i - 1]);
-    DevBuf dcp, dcoef, dout;
-    HIPCHK(hipMalloc(&dcp.p, cp.size() * 8));
-    HIPCHK(hipMalloc(&dcoef.p, (size_t)2 * n_rec * (nt + 2) * 8));
-    HIPCHK(hipMalloc(&dout.p, (size_t)2 * n_rec * nt_gf * 4));
-    HIPCHK(hipMemcpyAsync(dcp.p, cp.data(), cp.size() * 8, hipMemcpyHostToDevice, st));
-    const int ncol = (int)(2 * n_rec);
-    const void* rp = solver_trace_ptr(s, ADF_REC_P);
-    const void* rv = solver_trace_ptr(s, ADF_VN_REC);
-    if (s->dtype == ADF_F64)
-        hipLaunchKernelGGL((k_gf_coef<double>), dim3((ncol + 63) / 64), dim3(64), 0, st, (const double*)rp,
-                           (const double*)rv, (int)nt, (int)first, (int)n_rec, (const double*)dcp.p, (double*)dcoef.p);
-    else
-        hipLaunchKernelGGL((k_gf_coef<float>), dim3((ncol + 63) / 64), dim3(64), 0, st, (const float*)rp,
-                           (const float*)rv, (int)nt, (int)first, (int)n_rec, (const double*)dcp.p, (double*)dcoef.p);
     // t_shifted = range(-timeshift, tmax - timeshift; length = nt),
     // t_gf = range(0, tmax; length = nt_gf)
     const double tmax = s->dom.tmax;
     const double dt_in = nt > 1 ? tmax / (double)(nt - 1) : 1.0;
     const double dt_out = nt_gf > 1 ? tmax / (double)(nt_gf - 1) : 1.0;
     const double scale = dt_out / dt_in;
-    const int64_t tot = (int64_t)ncol * nt_gf;
-    hipLaunchKernelGGL(k_gf_eval, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, (const double*)dcoef.p,
-                       (int)nt, ncol, -timeshift, dt_in, 0.0, dt_out, (int)nt_gf, scale, (float*)dout.p);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out_p, dout.p, (size_t)n_rec * nt_gf * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(out_vn, (float*)dout.p + (size_t)n_rec * nt_gf, (size_t)n_rec * nt_gf * 4,
-                          hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    std::vector<float> out((size_t)2 * n_rec * nt_gf);
+    const void* rp = solver_trace_ptr(s, ADF_REC_P);
+    const void* rv = solver_trace_ptr(s, ADF_VN_REC);
+    int rc;
+    if (s->dtype == ADF_F64)
+        rc = resample_columns<double>(st, (const double*)rp, (const double*)rv, first, n_rec, nt, 2 * n_rec, 1,
+                                      -timeshift, dt_in, 0.0, dt_out, nt_gf, scale, out.data());
+    else
+        rc = resample_columns<float>(st, (const float*)rp, (const float*)rv, first, n_rec, nt, 2 * n_rec, 1,
+                                     -timeshift, dt_in, 0.0, dt_out, nt_gf, scale, out.data());
+    if (rc) return rc;
+    std::copy(out.begin(), out.begin() + n_rec * nt_gf, out_p);
+    std::copy(out.begin() + n_rec * nt_gf, out.end(), out_vn);
     return ADF_OK;
+}
+
+extern "C" int adf_resample_f32(adf_ctx* ctx, const float* in, int64_t n_in, int64_t ncol, double t_in0,
+                                double dt_in, double t_out0, double dt_out, int64_t n_out, double amp_scale,
+                                int denoise, float* out) {
+    if (!ctx) return fail(ADF_EINVAL, "adf_resample_f32: null context");
+    if (!in || !out || n_in < 1 || ncol < 1 || n_out < 1)
+        return fail(ADF_EINVAL, "adf_resample_f32: empty or missing input/output");
+    if (!(dt_in > 0.0) || !(dt_out > 0.0)) return fail(ADF_EINVAL, "adf_resample_f32: dt_in and dt_out must be > 0");
+    if (ncol * (n_in + 2) > ((int64_t)1 << 31) || ncol * n_out > ((int64_t)1 << 31) || ncol > (1 << 30))
+        return fail(ADF_EINVAL, "adf_resample_f32: too many samples");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    DevBuf din;
+    HIPCHK(hipMalloc(&din.p, (size_t)ncol * n_in * 4));
+    HIPCHK(hipMemcpyAsync(din.p, in, (size_t)ncol * n_in * 4, hipMemcpyHostToDevice, st));
+    return resample_columns<float>(st, (const float*)din.p, (const float*)din.p, 0, ncol, n_in, ncol, denoise ? 1 : 0,
+                                   t_in0, dt_in, t_out0, dt_out, n_out, amp_scale, out);
 }

@@ -289,6 +289,20 @@ int adf_batch_last_step_ms(adf_batch* b, double* ms);
 int adf_solver_gf_resample(adf_solver* s, int64_t first, int64_t n_rec, double timeshift, int64_t nt_gf,
                            float* out_p, float* out_vn);
 
+/* Cubic resampling of host float32 traces on the device, the reverb
+ * stage's per-source resample (src/reverb.jl:424-452 _resample_iSrc!,
+ * src/resample.jl:35-77 resample_time(kind = :cubic)): in holds ncol
+ * traces of n_in samples (column-major (n_in, ncol)) on t_in = t_in0 +
+ * (k-1) dt_in; each is interpolated by the natural cubic B-spline
+ * (Interpolations' BSpline(Cubic(Line(OnGrid())))), Flat outside, sampled
+ * at t_out = t_out0 + (k-1) dt_out, k = 1..n_out, multiplied by amp_scale
+ * (1 for scale_amplitude = false) and rounded to float32 into out
+ * (column-major (n_out, ncol)).  denoise = 1 first zeroes the samples below
+ * 1e-4 of each trace's signed maximum (src/greens.jl:176-177).  The spline
+ * is solved and evaluated in double.  Synchronous on ctx's stream. */
+int adf_resample_f32(adf_ctx* ctx, const float* in, int64_t n_in, int64_t ncol, double t_in0, double dt_in,
+                     double t_out0, double dt_out, int64_t n_out, double amp_scale, int denoise, float* out);
+
 /* --------------------------------------------------------- post-hoc KH
  *
  * Analytic Kirchhoff-Helmholtz kernels evaluated on the device and
