@@ -106,7 +106,7 @@ def make_batch_solver(sp, dtype=np.float32):
 def algorithmic_bytes_per_step(n, p, es, masked=True):
     """8 n^3 field words read+written, the CPML memory touched (read+write),
     and a 1-bit scatterer bitmap."""
-    n_psi = 3 * ((2 * p - 1) * (n - 1) ** 2 + (2 * p + 1) * n ** 2)
+    n_psi = 3 * ((2 * p - 1) * (n - 1) ** 2 + (2 * p + 1) * n ** 2) if p > 0 else 0
     return es * (8 * n ** 3 + 2 * n_psi) + (n ** 3 / 8 if masked else 0)
 
 
@@ -305,6 +305,35 @@ def gfs_leg(n_src=32, n=128, nt=1200, n_rec=100, nt_save=600):
             "p_p_peak": float(np.abs(g["p_p"]).max())}
 
 
+def reverb_leg(n_ill=32, n=128, nt=1200, n_rec=100):
+    """build_reverb_data end to end (src/reverb.jl:72-417): n_ill pressure
+    monopoles on the R2 illumination sphere (r = 56 cells) in a closed
+    128^3 box (npml = 0) around the sphere scatterer (r = 16 cells), p and
+    vn recorded on outer (r = 40 cells) and inner (r = 20 cells) Fibonacci
+    surfaces of n_rec points each, resampled to fs_out = 2 * 2.5 * fc.
+    Fresh sources advance as one batch.  Wall time of the whole call, set-up
+    and host transfers included."""
+    import adfdtd as A
+
+    dx = C_AIR / (2.5 * FC) / 10
+    xmax = (n - 1) * dx / 2
+    d0 = A.make_domain(1.0, xmax, xmax, xmax, n, n, n, C_AIR, RHO_AIR, 0.5)
+    d = A.make_domain((nt - 0.5) * d0.dt, xmax, xmax, xmax, n, n, n, C_AIR, RHO_AIR, 0.5)
+    cfg = {"reverb": {"radius_ill": 56 * d.dx, "fc_source": FC, "bandwidth_factor": 2.5, "nPoints_ill": n_ill},
+           "pml": {"n": 0, "rcoef": 1e-4, "fc": FC},
+           "surfaces": {"nPoints_outer": n_rec, "radius_outer": 40 * d.dx, "nPoints_inner": n_rec,
+                        "radius_inner": 20 * d.dx, "sphere_radius": 16 * d.dx}}
+    t0 = time.perf_counter()
+    g = A.build_reverb_data(d, cfg, scatterer="sphere", save_field=False, sources_per_batch=n_ill)
+    wall = time.perf_counter() - t0
+    byts = n_ill * algorithmic_bytes_per_step(n, 0, 4)
+    return {"sources": n_ill, "grid": n, "steps": nt, "receivers": 2 * n_rec, "nt_ill": int(g["t"].shape[0]),
+            "wall_s": round(wall, 3), "sources_per_s": round(n_ill / wall, 3),
+            "glups_wall": round(n_ill * n ** 3 * nt / wall / 1e9, 2),
+            "hbm_frac_wall": round(byts * nt / wall / 1e9 / HBM_PEAK_GBS, 4),
+            "inner_p_peak": float(np.abs(g["inner_p"]).max())}
+
+
 def kh_leg(which="p_v"):
     """Post-hoc KH of the 256^3 workload's surfaces: the 2000-point KH sphere
     (r = 40 cells) extrapolated to the 500 far-field points (r = 80 cells)
@@ -496,6 +525,10 @@ def main():
                                             "128^3 FP32, sphere r=16 cells, 100 inner receivers, 1200 steps, "
                                             "resampled to 600 samples", **gfs_leg()}
         log("impulsive_gfs leg done")
+        res["reverb"] = {"workload": "build_reverb_data end to end: 32 R2 illumination sources, closed 128^3 box "
+                                     "(npml 0), sphere r=16 cells, 100 outer + 100 inner receivers, 1200 steps, "
+                                     "resampled to fs_out = 5 fc", **reverb_leg()}
+        log("reverb leg done")
         res["kh_post_hoc"] = {"workload": "analytic KH p_v kernel, 2000-point KH sphere -> 500 far-field points, "
                                           "2000 steps, FP32 kernel samples and inputs, double sums", **kh_leg()}
         log("post-hoc KH leg done")
