@@ -34,31 +34,39 @@ template <typename T, bool EXACT, int CPL, bool PF>
 __global__ __launch_bounds__(256) void k_sweep_batch(const Fused<T>* __restrict__ fs, const int* __restrict__ cur,
                                                      const uint2* __restrict__ tiles, int ntiles, unsigned* ctr,
                                                      int persistent) {
-    auto run = [&](int t) {
+    // One call site of the tile routine for both modes: two would be
+    // compiled as two copies (or as a call with a stack frame).
+    const int home = (int)(blockIdx.x & 7);
+    int q = 0;  // persistent: queues (home + q) & 7 drained so far
+    for (bool first = true;; first = false) {
+        int t;
+        if (!persistent) {
+            if (!first) break;
+            const int W = (int)(blockDim.x >> 6);
+            t = xcd_tile((int)blockIdx.x, (int)gridDim.x) * W + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+            if (t >= ntiles) break;
+        } else {
+            t = -1;
+            while (q < 8) {
+                const int x = (home + q) & 7;
+                unsigned v = 0;
+                if ((threadIdx.x & 63) == 0) v = atomicAdd(&ctr[x], 1u);
+                // wave-uniform: keep the tile index and the per-solve loads scalar
+                const int u = __builtin_amdgcn_readfirstlane((int)v);
+                if (u < xcd_chunk_size(x, ntiles)) {
+                    t = xcd_chunk_start(x, ntiles) + u;
+                    break;
+                }
+                ++q;
+            }
+            if (t < 0) break;
+        }
         const uint2 e = tiles[t];
         const int s = (int)((e.x >> 16) & 0xfffu);
         const Fused<T> F = fs[2 * s + cur[s]];
         sweep_tile<T, EXACT, CPL, PF>(F, e);
-    };
-    if (!persistent) {
-        const int W = (int)(blockDim.x >> 6);
-        const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x) * W + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-        if (t < ntiles) run(t);
-        return;
     }
-    const int home = (int)(blockIdx.x & 7);
-    for (int q = 0; q < 8; ++q) {
-        const int x = (home + q) & 7;
-        const int n = xcd_chunk_size(x, ntiles), o = xcd_chunk_start(x, ntiles);
-        for (;;) {
-            unsigned v = 0;
-            if ((threadIdx.x & 63) == 0) v = atomicAdd(&ctr[x], 1u);
-            // wave-uniform: keep the tile index and the per-solve loads scalar
-            const int t = __builtin_amdgcn_readfirstlane((int)v);
-            if (t >= n) break;
-            run(o + t);
-        }
-    }
+    if (!persistent) return;
     if ((threadIdx.x & 63) == 0) {
         __threadfence();
         const unsigned d = atomicAdd(&ctr[8], 1u);
@@ -82,7 +90,7 @@ __global__ __launch_bounds__(256) void k_tx_p_batch(const TxDev<T>* __restrict__
     const int is = (int)(e & 0x7ffffu);
     const TxDev<T> D = td[s];
     const Txs<T>& X = D.X;
-    T* P = D.fb[cur[s] ^ 1];
+    T* P = (cur[s] ^ 1) ? D.fb[1] : D.fb[0];  // a select: a runtime index would put D on the stack
     const size_t o = (size_t)is * X.nt + it0s[s];
     tx_corners<T>(P, X.idx + 3 * is, D.nx, D.ny, D.nz, X.w + 8 * is, X.rec + o, X.src + o, ((e >> 19) & 1u) != 0,
                   live, q, g0);
@@ -108,7 +116,7 @@ __global__ __launch_bounds__(256) void k_tx_v_batch(const TxDev<T>* __restrict__
     const size_t o = (size_t)is * nt + it0;
     const size_t cs = (size_t)ntx * nt;
     const int c1 = (ch < 3 ? ch : 2) + 1;
-    T* F = D.fb[cur[s] ^ 1] + (size_t)c1 * D.n3;
+    T* F = ((cur[s] ^ 1) ? D.fb[1] : D.fb[0]) + (size_t)c1 * D.n3;
     const T r = tx_corners<T>(F, X.idx + 3 * (is + ntx * c1), D.nx, D.ny, D.nz, X.w + 8 * (is + ntx * c1),
                               X.rec + o + cs * c1, X.src + o + cs * c1, inj, live, q, g0);
     const T rx = __shfl(r, g32, 64), ry = __shfl(r, g32 + 8, 64), rz = __shfl(r, g32 + 16, 64);
@@ -156,7 +164,7 @@ __global__ void k_mirror_batch(const TxDev<T>* __restrict__ td, const int* __res
                                const int* __restrict__ solves) {
     const int s = solves[blockIdx.y];
     const TxDev<T> D = td[s];
-    T* F = D.fb[cur[s] ^ 1];
+    T* F = (cur[s] ^ 1) ? D.fb[1] : D.fb[0];
     T* Vy = F + 2 * (size_t)D.n3;
     T* Vz = F + 3 * (size_t)D.n3;
     const int nx = D.nx, ny = D.ny, nz = D.nz;

@@ -134,7 +134,9 @@ __device__ __forceinline__ void bld(Rsrc r, int voff, int soff, T (&o)[N]) {
 // them before it).  Without this, 256^3 float32 sweeps stored, in some
 // lanes, a value that a later VALU or LDS instruction wrote into the first
 // data register of such a store (results varying from run to run); the
-// compiler inserts no wait state after stores with an SGPR offset.
+// compiler inserts no wait state after stores with an SGPR offset.  The
+// same hazard in compiler-written spill code is why no kernel here may use
+// scratch memory (the Makefile checks it).
 template <int POL>
 __device__ __forceinline__ void store_b128(u32x4 u, Rsrc r, int voff, int soff) {
     __builtin_amdgcn_raw_buffer_store_b128(u, r, voff, soff, POL);

@@ -108,6 +108,14 @@ struct adf_solver {
     int64_t gf_nr = 0, gf_ns = 0;
     double* gf_h0 = nullptr;
     double* gf_h1 = nullptr;
+    // blocked gather: far-pass partial sums [chunk][GF_M][2][n_isrc], the
+    // steps [gf_B, gf_Bend) (0-based) they cover, and the step after the
+    // last one run (a run continuing from it keeps the current block)
+    double* gf_part = nullptr;
+    int gf_nchunk = 0;
+    int64_t gf_max_index = 0;          // largest 1-based iRec/iSrc of the map
+    std::vector<int64_t> gf_isrc_h;    // the map's iSrc (1-based), host copy
+    int64_t gf_B = -1, gf_Bend = -1, gf_next = -1;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_step_ms = 0.0;

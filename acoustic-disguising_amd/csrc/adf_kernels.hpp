@@ -356,52 +356,161 @@ __global__ void k_gf_history(Txs<T> X, const int32_t* iRec, int nr, int kind, do
     }
 }
 
-// One output column per block: the gather of extrapolation.jl:101-192
-// (sum over inputs r and lags k < Tn of G[k, ., .] * hist[r][it0-k]),
-// written into the source slots at it+1 (it0+1 0-based).
-template <typename T, typename G>
-__global__ __launch_bounds__(256) void k_gf_gather(Txs<T> X, int kind, int writeback, const G* g0,
-                                                   const G* g1, const G* g2, const G* g3, int ng,
-                                                   int d2, int d3, const int32_t* iSrc, int nout,
-                                                   int nin, const double* h0, const double* h1,
-                                                   double z0, int it0) {
-    const int s = blockIdx.x;
-    if (s >= nout) return;
-    const int Tn = min(it0 + 1, ng);
-    double acc0 = 0.0, acc1 = 0.0;
-    for (int r = 0; r < nin; ++r) {
-        size_t base;
-        if (kind == 1) base = (size_t)ng * (s + (size_t)d2 * r);       // (ng, nsrc, nrec)
-        else if (kind == 3) base = (size_t)ng * (r + (size_t)d2 * s);  // (ng, nrec, nsrc)
-        else base = (size_t)ng * (r + (size_t)d2 * s);                 // (ng, nm, nout)
-        const double* hr0 = h0 + (size_t)r * X.nt + it0;
-        const double* hr1 = kind == 2 ? h1 + (size_t)r * X.nt + it0 : nullptr;
-        for (int k = threadIdx.x; k < Tn; k += blockDim.x) {
-            const double x0 = hr0[-k];
-            if (kind == 2) {
-                const double x1 = hr1[-k];
-                acc0 += (double)g1[base + k] * x0 + (double)g0[base + k] * x1;  // pv*p + pp*vn
-                acc1 += (double)g3[base + k] * x0 + (double)g2[base + k] * x1;  // vv*p + vp*vn
-            } else {
-                acc0 += (double)g0[base + k] * x0;
-                acc1 += (double)g1[base + k] * x0;
+// The gather of extrapolation.jl:101-192: for each output s at step t
+// (0-based; written into the source slots at t+1)
+//   out(t)[s] = sum_r sum_{k=0}^{min(t, ng-1)} G[k, s, r] * hist[r][t-k]
+// (two outputs, and for PVFromPV two histories and four tensors).  Every
+// element of G is used once per step, so a step-by-step gather reads the
+// whole map every step.  It is split in time instead: a far pass, once per
+// block of GF_M steps starting at step B, sums for every step t = B+m of
+// the block the terms whose history lies before the block (lags
+// k >= m+1), reading each element of G once for all GF_M steps; a near
+// pass at each step adds the terms on the block's own history (lags
+// k <= m) and the far partial sums, in a fixed order.
+constexpr int GF_M = 16;    // steps per far pass
+constexpr int GF_KT = 4;    // consecutive lags per thread in the far pass
+constexpr int GF_RC = 16;   // inputs per far-pass workgroup
+constexpr int GF_KTILE = 256 * GF_KT;
+
+// Element offset of G[0, s, r]: PVFromPin (ng, n_isrc, n_irec); PVFromPV and
+// PFromPin (ng, n_irec, n_isrc).
+__device__ __forceinline__ size_t gf_base(int kind, int s, int r, int ng, int d2) {
+    return kind == 1 ? (size_t)ng * ((size_t)s + (size_t)d2 * r) : (size_t)ng * ((size_t)r + (size_t)d2 * s);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Far pass of the block [B, B+Mb): output s = blockIdx.x, inputs of chunk
+// blockIdx.y; part[((c * GF_M + m) * 2 + ch) * nout + s].  The history of
+// each input is staged in LDS over the lag tile's time window with zeros
+// at times >= B (the near pass's terms) and < 0 (lags beyond t).
+template <int KIND, typename G>
+__global__ __launch_bounds__(256) void k_gf_far(const G* __restrict__ g0, const G* __restrict__ g1,
+                                                const G* __restrict__ g2, const G* __restrict__ g3, int ng,
+                                                int d2, int nin, int nout, const double* __restrict__ h0,
+                                                const double* __restrict__ h1, int nt, int B, int Mb,
+                                                double* __restrict__ part) {
+    constexpr int NW = KIND == 2 ? 2 : 1;
+    constexpr int WIN = GF_KTILE + GF_M - 1;
+    __shared__ double win[NW][WIN];
+    __shared__ double red[4][2 * GF_M];
+    const int s = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+    const int r0 = c * GF_RC, r1 = min(r0 + GF_RC, nin);
+    const int kmax = min(ng - 1, B + Mb - 1);
+    double acc0[GF_M], acc1[GF_M];
+#pragma unroll
+    for (int m = 0; m < GF_M; ++m) acc0[m] = acc1[m] = 0.0;
+    for (int r = r0; r < r1; ++r) {
+        const size_t gb = gf_base(KIND, s, r, ng, d2);
+        for (int k0 = 0; k0 <= kmax; k0 += GF_KTILE) {
+            // win[i] = hist[r][B - k0 - (GF_KTILE-1) + i] at times in [0, B)
+            const int tw = B - k0 - (GF_KTILE - 1);
+            __syncthreads();
+            for (int i = tid; i < WIN; i += 256) {
+                const int tt = tw + i;
+                const bool in = tt >= 0 && tt < B;
+                win[0][i] = in ? h0[(size_t)r * nt + tt] : 0.0;
+                if constexpr (NW == 2) win[NW - 1][i] = in ? h1[(size_t)r * nt + tt] : 0.0;
+            }
+            __syncthreads();
+            // lags k0 + KT*tid + j need win[base + 3 - j + m], base = KTILE-KT - KT*tid
+            const int wb = GF_KTILE - GF_KT - GF_KT * tid;
+            double wp[GF_M + GF_KT - 1], wv[NW == 2 ? GF_M + GF_KT - 1 : 1];
+#pragma unroll
+            for (int q = 0; q < GF_M + GF_KT - 1; ++q) {
+                wp[q] = win[0][wb + q];
+                if constexpr (NW == 2) wv[q] = win[NW - 1][wb + q];
+            }
+#pragma unroll
+            for (int j = 0; j < GF_KT; ++j) {
+                const int k = k0 + GF_KT * tid + j;
+                const bool ok = k <= kmax;
+                const double a = ok ? (double)g0[gb + k] : 0.0, b = ok ? (double)g1[gb + k] : 0.0;
+                if constexpr (KIND == 2) {
+                    const double e = ok ? (double)g2[gb + k] : 0.0, f = ok ? (double)g3[gb + k] : 0.0;
+#pragma unroll
+                    for (int m = 0; m < GF_M; ++m) {
+                        const double xp = wp[GF_KT - 1 - j + m], xv = wv[GF_KT - 1 - j + m];
+                        acc0[m] += b * xp + a * xv;  // pv*p + pp*vn
+                        acc1[m] += f * xp + e * xv;  // vv*p + vp*vn
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < GF_M; ++m) {
+                        const double x = wp[GF_KT - 1 - j + m];
+                        acc0[m] += a * x;
+                        acc1[m] += b * x;
+                    }
+                }
             }
         }
     }
-    __shared__ double red[2][256];
-    red[0][threadIdx.x] = acc0;
-    red[1][threadIdx.x] = acc1;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + w];
-            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    // block sums of the 2*GF_M accumulators
+    const int w = tid >> 6, l = tid & 63;
+#pragma unroll
+    for (int m = 0; m < GF_M; ++m) {
+        const double s0 = wave_sum(acc0[m]), s1 = wave_sum(acc1[m]);
+        if (l == 0) {
+            red[w][2 * m] = s0;
+            red[w][2 * m + 1] = s1;
         }
-        __syncthreads();
     }
-    if (threadIdx.x == 0 && writeback && it0 + 1 < X.nt) {
-        const size_t o = (size_t)iSrc[s] * X.nt + it0 + 1;
-        const double a0 = red[0][0], a1 = red[1][0];
+    __syncthreads();
+    if (tid < 2 * GF_M) {
+        const int m = tid >> 1, ch = tid & 1;
+        const double v = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+        if (m < Mb) part[((size_t)(c * GF_M + m) * 2 + ch) * nout + s] = v;
+    }
+}
+
+// Near pass at step t = B + m: lags 0..min(m, t, ng-1) of every input, plus
+// the far partial sums of the nchunk input chunks, written into the source
+// slots at t+1.
+template <typename T, typename G>
+__global__ __launch_bounds__(256) void k_gf_near(Txs<T> X, int kind, int writeback, const G* __restrict__ g0,
+                                                 const G* __restrict__ g1, const G* __restrict__ g2,
+                                                 const G* __restrict__ g3, int ng, int d2,
+                                                 const int32_t* __restrict__ iSrc, int nout, int nin,
+                                                 const double* __restrict__ h0, const double* __restrict__ h1,
+                                                 double z0, int t, int m, int nchunk,
+                                                 const double* __restrict__ part) {
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const int kn = min(m, min(t, ng - 1));
+    double acc0 = 0.0, acc1 = 0.0;
+    for (int r = tid; r < nin; r += 256) {
+        const size_t gb = gf_base(kind, s, r, ng, d2);
+        const double* hr0 = h0 + (size_t)r * X.nt + t;
+        for (int k = 0; k <= kn; ++k) {
+            const double x0 = hr0[-k];
+            if (kind == 2) {
+                const double x1 = h1[(size_t)r * X.nt + t - k];
+                acc0 += (double)g1[gb + k] * x0 + (double)g0[gb + k] * x1;  // pv*p + pp*vn
+                acc1 += (double)g3[gb + k] * x0 + (double)g2[gb + k] * x1;  // vv*p + vp*vn
+            } else {
+                acc0 += (double)g0[gb + k] * x0;
+                acc1 += (double)g1[gb + k] * x0;
+            }
+        }
+    }
+    __shared__ double red[2][4];
+    const double s0 = wave_sum(acc0), s1 = wave_sum(acc1);
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = s0;
+        red[1][tid >> 6] = s1;
+    }
+    __syncthreads();
+    if (tid == 0 && writeback && t + 1 < X.nt) {
+        double a0 = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        double a1 = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+        for (int c = 0; c < nchunk; ++c) {
+            a0 += part[((size_t)(c * GF_M + m) * 2) * nout + s];
+            a1 += part[((size_t)(c * GF_M + m) * 2 + 1) * nout + s];
+        }
+        const size_t o = (size_t)iSrc[s] * X.nt + t + 1;
         if (kind == 1 || kind == 2) {
             // p_src <- vp-derived, vn_src <- pp-derived (extrapolation.jl:165-167,215-216)
             X.src[o] = (T)a1;

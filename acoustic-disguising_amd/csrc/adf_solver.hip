@@ -121,10 +121,13 @@ static void free_txs(adf_solver* s) {
 
 static void free_gf(adf_solver* s) {
     for (int i = 0; i < 4; ++i) { dfree(s->gf_t[i]); s->gf_t[i] = nullptr; }
-    dfree(s->gf_irec); dfree(s->gf_isrc); dfree(s->gf_h0); dfree(s->gf_h1);
+    dfree(s->gf_irec); dfree(s->gf_isrc); dfree(s->gf_h0); dfree(s->gf_h1); dfree(s->gf_part);
     s->gf_irec = s->gf_isrc = nullptr;
-    s->gf_h0 = s->gf_h1 = nullptr;
+    s->gf_h0 = s->gf_h1 = s->gf_part = nullptr;
     s->gf_kind = 0;
+    s->gf_B = s->gf_Bend = s->gf_next = -1;
+    s->gf_max_index = 0;
+    s->gf_isrc_h.clear();
 }
 
 void solver_free(adf_solver* s) {
@@ -144,6 +147,7 @@ void solver_free(adf_solver* s) {
 }
 
 int solver_zero_state(adf_solver* s) {
+    s->gf_B = s->gf_Bend = s->gf_next = -1;  // the blocked gather starts over
     hipStream_t st = s->ctx->stream;
     for (int b = 0; b < 2; ++b) {
         HIPCHK(hipMemsetAsync(s->fieldb[b], 0, 4 * s->n3 * s->es, st));
@@ -308,6 +312,10 @@ int rebuild_schedule(adf_solver* s) {
 
 int solver_set_txs(adf_solver* s, int64_t ntx, const int64_t* idx, const void* w, const void* src,
                    const void* dir, const void* vn_src) {
+    if (s->gf_kind && s->gf_max_index > ntx)
+        return fail(ADF_EINVAL, "adf_solver_set_transceivers: the GF map's iRec/iSrc reach transceiver " +
+                                    std::to_string(s->gf_max_index) + " of " + std::to_string(ntx) +
+                                    "; remove the map (adf_solver_set_gf_map kind 0) or set a new one first");
     hipStreamSynchronize(s->ctx->stream);
     free_txs(s);
     if (ntx == 0) {
@@ -350,6 +358,9 @@ int solver_set_txs(adf_solver* s, int64_t ntx, const int64_t* idx, const void* w
     for (int c = 0; c < 3; ++c) s->src_inj_v[c].assign(ntx, 0);
     s->vn_inj.assign(ntx, 0);
     s->gf_inj.assign(ntx, 0);
+    // a GF map set before keeps writing its sources: they stay injectors
+    if (s->gf_kind && s->gf_wb)
+        for (int64_t i : s->gf_isrc_h) s->gf_inj[i - 1] = 1;
     const size_t chb = (size_t)ntx * nt * es;
     for (int c = 0; c < 4; ++c) injectors_from(s, c, (const char*)src + c * chb);
     if (vn_src) injectors_from(s, 4, vn_src);
@@ -430,6 +441,14 @@ int solver_set_gf(adf_solver* s, int kind, const double* const* tensors, int64_t
     s->gf_d3 = dim3;
     s->gf_nr = n_irec;
     s->gf_ns = n_isrc;
+    s->gf_nchunk = (int)((n_irec + GF_RC - 1) / GF_RC);
+    s->gf_isrc_h.assign(iSrc, iSrc + n_isrc);
+    s->gf_max_index = 0;
+    for (int64_t i = 0; i < n_irec; ++i) s->gf_max_index = std::max(s->gf_max_index, iRec[i]);
+    for (int64_t i = 0; i < n_isrc; ++i) s->gf_max_index = std::max(s->gf_max_index, iSrc[i]);
+    if ((rc = dmalloc((void**)&s->gf_part, (size_t)s->gf_nchunk * GF_M * 2 * n_isrc * 8, "GF partial sums")))
+        return rc;
+    s->gf_B = s->gf_Bend = s->gf_next = -1;
     return ADF_OK;
 }
 
@@ -585,6 +604,35 @@ static void launch_tx_p(adf_solver* s, const Solve<T>& S, const Txs<T>& X, int i
     }
 }
 
+// The blocked GF gather of step t (0-based; the history of t is recorded):
+// a far pass when t starts a block (a multiple of GF_M, or the first step
+// of a run that does not continue the previous one), then the near pass.
+template <typename T>
+static void launch_gf_gather(adf_solver* s, const Txs<T>& X, int t, hipStream_t st) {
+    const int nt = (int)s->dom.nt, ng = (int)s->gf_ng, d2 = (int)s->gf_d2;
+    const int nin = (int)s->gf_nr, nout = (int)s->gf_ns;
+    const T* g[4] = {(const T*)s->gf_t[0], (const T*)s->gf_t[1], (const T*)s->gf_t[2], (const T*)s->gf_t[3]};
+    if (!(s->gf_B >= 0 && t >= s->gf_B && t < s->gf_Bend)) {
+        const int64_t B = t, Bend = std::min<int64_t>((t / GF_M + 1) * (int64_t)GF_M, nt - 1);
+        const dim3 grid((unsigned)nout, (unsigned)s->gf_nchunk);
+        const int Mb = (int)(Bend - B);
+        if (s->gf_kind == 2)
+            hipLaunchKernelGGL((k_gf_far<2, T>), grid, dim3(256), 0, st, g[0], g[1], g[2], g[3], ng, d2, nin, nout,
+                               s->gf_h0, s->gf_h1, nt, (int)B, Mb, s->gf_part);
+        else if (s->gf_kind == 1)
+            hipLaunchKernelGGL((k_gf_far<1, T>), grid, dim3(256), 0, st, g[0], g[1], g[2], g[3], ng, d2, nin, nout,
+                               s->gf_h0, s->gf_h1, nt, (int)B, Mb, s->gf_part);
+        else
+            hipLaunchKernelGGL((k_gf_far<3, T>), grid, dim3(256), 0, st, g[0], g[1], g[2], g[3], ng, d2, nin, nout,
+                               s->gf_h0, s->gf_h1, nt, (int)B, Mb, s->gf_part);
+        s->gf_B = B;
+        s->gf_Bend = Bend;
+    }
+    hipLaunchKernelGGL((k_gf_near<T, T>), dim3((unsigned)nout), dim3(256), 0, st, X, s->gf_kind, s->gf_wb, g[0], g[1],
+                       g[2], g[3], ng, d2, s->gf_isrc, nout, nin, s->gf_h0, s->gf_h1, s->dom.z0, t,
+                       (int)(t - s->gf_B), s->gf_nchunk, (const double*)s->gf_part);
+}
+
 template <typename T>
 static void launch_after_v(adf_solver* s, const Solve<T>& S, const Txs<T>& X, int64_t it, hipStream_t st) {
     const int it0 = (int)(it - 1);
@@ -602,11 +650,7 @@ static void launch_after_v(adf_solver* s, const Solve<T>& S, const Txs<T>& X, in
             hipLaunchKernelGGL((k_gf_history<T>), dim3((nr + 255) / 256), dim3(256), 0, st, X, s->gf_irec, nr,
                                s->gf_kind, s->dom.z0, s->gf_h0, s->gf_h1, it0);
             if (it < nt) {
-                const T* g[4] = {(const T*)s->gf_t[0], (const T*)s->gf_t[1], (const T*)s->gf_t[2],
-                                 (const T*)s->gf_t[3]};
-                hipLaunchKernelGGL((k_gf_gather<T, T>), dim3((unsigned)s->gf_ns), dim3(256), 0, st, X, s->gf_kind,
-                                   s->gf_wb, g[0], g[1], g[2], g[3], (int)s->gf_ng, (int)s->gf_d2, (int)s->gf_d3,
-                                   s->gf_isrc, (int)s->gf_ns, nr, s->gf_h0, s->gf_h1, s->dom.z0, it0);
+                if (s->gf_wb) launch_gf_gather<T>(s, X, it0, st);
                 hipLaunchKernelGGL((k_vn2v<T>), dim3((unsigned)((s->ntx + 255) / 256)), dim3(256), 0, st, X, it0);
             }
         }
@@ -866,6 +910,7 @@ static int run_impl(adf_solver* s, int64_t start, int64_t end) {
     s->sweep_avg_ms = 0.0;
     s->sweep_n = 0;
     s->sweep_seq = 0;
+    if (start - 1 != s->gf_next) s->gf_B = s->gf_Bend = -1;  // not a continuation: new gather block
     HIPCHK(hipEventRecord(s->ev0, st));
     for (int64_t it = start; it <= end; ++it) {
         step_impl<T, EXACT>(s, it, st, fused, ty);
@@ -882,6 +927,7 @@ static int run_impl(adf_solver* s, int64_t start, int64_t end) {
         }
     }
     HIPCHK(hipGetLastError());
+    s->gf_next = end;
     HIPCHK(hipEventRecord(s->ev1, st));
     HIPCHK(hipEventSynchronize(s->ev1));
     float ms = 0.f;

@@ -334,6 +334,49 @@ def reverb_leg(n_ill=32, n=128, nt=1200, n_rec=100):
             "inner_p_peak": float(np.abs(g["inner_p"]).max())}
 
 
+def gf_map_leg(n=192, nio=400, ng=827, steps=100):
+    """In-loop KH gather (extrapolation.jl:101-249) at paper scale: a 192^3
+    FP32 solve (12-cell CPML, cube r=30 cells) whose 400 outer receivers
+    (r = 80 cells) drive 400 inner sources (r = 40 cells) through a
+    PVFromPin map of nt_gf = 827 lags (random tensors).  Device time per
+    step with and without the map, after nt_gf steps so every step uses
+    the whole map; the gather reads 2 * 400 * 400 * 827 float32 per step's
+    worth of MACs."""
+    import adfdtd as A
+
+    dx = C_AIR / (2.5 * FC) / 10
+    xm = (n - 1) * dx / 2
+    nt = ng + steps + 60
+    d0 = A.make_domain(1.0, xm, xm, xm, n, n, n, C_AIR, RHO_AIR, 0.5)
+    d = A.make_domain((nt - 0.5) * d0.dt, xm, xm, xm, n, n, n, C_AIR, RHO_AIR, 0.5)
+    ro, rn = A.fibonacci_sphere(nio, 80 * d.dx)
+    io, inn = A.fibonacci_sphere(nio, 40 * d.dx)
+    z = np.zeros(d.nt)
+    txs = [A.make_transceiver(d, [60 * d.dx, 0, 0], [1, 0, 0, 0], A.ricker_wavelet(d, FC, True))]
+    txs += [A.make_transceiver(d, ro[i], np.r_[0.0, rn[i]], z) for i in range(nio)]
+    txs += [A.make_transceiver(d, io[i], [1.0, inn[i][0], inn[i][1], inn[i][2]], z) for i in range(nio)]
+    rng = np.random.default_rng(1)
+    pp = (rng.standard_normal((nio, nio, ng)) * 1e-3).astype(np.float32)
+    vp = (rng.standard_normal((nio, nio, ng)) * 1e-3).astype(np.float32)
+    out = {}
+    for use in (0, 1):
+        s = A.Solver(d, A.dtype_code(np.float32, "fast"), A.make_cpml(d, 12, 1e-4, FC, np.float32))
+        s.build_mask("cube", [30 * d.dx])
+        s.set_transceivers(A.transceivers_to_grid(d, txs, np.float32))
+        if use:
+            s.set_gf_map(A.PVFromPin(pp, vp, list(range(2, 2 + nio)), list(range(2 + nio, 2 + 2 * nio)), True))
+        s.run(1, ng + 40)
+        s.run(ng + 41, ng + 40 + steps)
+        out[use] = s.last_step_ms()
+        del s
+    g = out[1] - out[0]
+    macs = 2 * nio * nio * ng
+    return {"grid": n, "inputs": nio, "outputs": nio, "nt_gf": ng, "steps": steps,
+            "step_ms_without_map": round(out[0], 4), "step_ms_with_map": round(out[1], 4),
+            "gather_ms_per_step": round(g, 4), "gather_gmacs": round(macs / (g * 1e-3) / 1e9, 1),
+            "map_bytes": macs * 4}
+
+
 def kh_leg(which="p_v"):
     """Post-hoc KH of the 256^3 workload's surfaces: the 2000-point KH sphere
     (r = 40 cells) extrapolated to the 500 far-field points (r = 80 cells)
@@ -529,6 +572,9 @@ def main():
                                      "(npml 0), sphere r=16 cells, 100 outer + 100 inner receivers, 1200 steps, "
                                      "resampled to fs_out = 5 fc", **reverb_leg()}
         log("reverb leg done")
+        res["kh_in_loop"] = {"workload": "in-loop KH gather, 192^3 FP32 solve, 400 outer receivers -> 400 inner "
+                                         "sources through a PVFromPin map of 827 lags (paper scale)", **gf_map_leg()}
+        log("in-loop KH gather leg done")
         res["kh_post_hoc"] = {"workload": "analytic KH p_v kernel, 2000-point KH sphere -> 500 far-field points, "
                                           "2000 steps, FP32 kernel samples and inputs, double sums", **kh_leg()}
         log("post-hoc KH leg done")

@@ -179,9 +179,11 @@ static int FN(ref_extrapolate_step)(const ref_domain* d, const ref_gfmap* gf, in
                 }
             }
         } else {
-            /* extrapolate_p_from_pin (:172-192); pos,neg are (nt_gf, nrec, nsrc) */
-            const int64_t ns = gf->nsrc;
-            if (ns != gf->n_isrc || gf->nrec != nr) { free(pin); return -3; }
+            /* extrapolate_p_from_pin (:172-192); pos,neg are (nt_gf, nrec, nsrc), so
+             * their second dimension (held in gf->nsrc, named for the PVFromPin
+             * layout) is #iRec and the third (gf->nrec) is #iSrc */
+            const int64_t ns = gf->n_isrc;
+            if (gf->nsrc != nr || gf->nrec != ns) { free(pin); return -3; }
             for (int64_t s = 0; s < ns; ++s) {
                 double pos = 0.0, neg = 0.0;
                 for (int64_t r = 0; r < nr; ++r)
