@@ -103,6 +103,7 @@ struct adf_solver {
     int gf_wb = 1;
     int64_t gf_ng = 0, gf_d2 = 0, gf_d3 = 0;
     void* gf_t[4] = {nullptr, nullptr, nullptr, nullptr};
+    void* gf_tn[4] = {nullptr, nullptr, nullptr, nullptr};  // lags 0..GF_M-1 as [s][r][GF_M]
     int32_t* gf_irec = nullptr;
     int32_t* gf_isrc = nullptr;
     int64_t gf_nr = 0, gf_ns = 0;
@@ -112,6 +113,7 @@ struct adf_solver {
     // steps [gf_B, gf_Bend) (0-based) they cover, and the step after the
     // last one run (a run continuing from it keeps the current block)
     double* gf_part = nullptr;
+    int gf_rc = 1;
     int gf_nchunk = 0;
     int64_t gf_max_index = 0;          // largest 1-based iRec/iSrc of the map
     std::vector<int64_t> gf_isrc_h;    // the map's iSrc (1-based), host copy

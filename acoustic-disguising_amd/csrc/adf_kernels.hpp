@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 namespace adf {
 
 // Scalars and pointers of one solve.  Arrays follow the reference's Julia
@@ -366,11 +368,21 @@ __global__ void k_gf_history(Txs<T> X, const int32_t* iRec, int nr, int kind, do
 // the block the terms whose history lies before the block (lags
 // k >= m+1), reading each element of G once for all GF_M steps; a near
 // pass at each step adds the terms on the block's own history (lags
-// k <= m) and the far partial sums, in a fixed order.
-constexpr int GF_M = 16;    // steps per far pass
-constexpr int GF_KT = 4;    // consecutive lags per thread in the far pass
-constexpr int GF_RC = 16;   // inputs per far-pass workgroup
-constexpr int GF_KTILE = 256 * GF_KT;
+// k <= m) and the far sums, in a fixed order.
+//
+// The far pass is a product of a GF_M x K Toeplitz matrix of history
+// values, A[m][(r,k)] = hist[r][B+m-k] for B+m-k in [0, B) and 0 elsewhere,
+// with the K x n_out map, K = (input, lag): it runs on the FP64 matrix
+// cores (v_mfma_f64_16x16x4f64: M = the 16 steps of the block, N = 16
+// outputs, 4 lags per instruction).
+constexpr int GF_M = 16;         // steps per far pass = the MFMA's M
+constexpr int GF_FAR_WAVES = 8192;  // far-pass waves aimed for (inputs per wave are chosen to reach it)
+
+// Inputs per far-pass wave for a map of nin inputs and nout outputs.
+inline int gf_far_rc(int64_t nin, int64_t nout) {
+    const int64_t n_og = (nout + 15) / 16;
+    return (int)std::max<int64_t>(1, (nin * n_og + GF_FAR_WAVES - 1) / GF_FAR_WAVES);
+}
 
 // Element offset of G[0, s, r]: PVFromPin (ng, n_isrc, n_irec); PVFromPV and
 // PFromPin (ng, n_irec, n_isrc).
@@ -384,119 +396,154 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Far pass of the block [B, B+Mb): output s = blockIdx.x, inputs of chunk
-// blockIdx.y; part[((c * GF_M + m) * 2 + ch) * nout + s].  The history of
-// each input is staged in LDS over the lag tile's time window with zeros
-// at times >= B (the near pass's terms) and < 0 (lags beyond t).
+typedef double gf_d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ gf_d4 mfma_f64(double a, double b, gf_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Far pass of the block [B, B+Mb).  One wave per (16 outputs, chunk of rc
+// inputs): wave w takes output group w % n_og and chunk w / n_og.  Lane l
+// holds A[m = l%16][kk = l/16] and B[kk = l/16][n = l%16]; instruction j of
+// a 16-lag step k0 puts lag k0 + 4*kk + j at K index kk (A and B agree, so
+// the sum is over all 16 lags).  The accumulator holds D[4v + l/16][l%16] in
+// element v.  Writes part[((c * GF_M + m) * 2 + ch) * nout + s] for m < Mb.
+// No LDS and no barriers: each wave runs on its own.
 template <int KIND, typename G>
 __global__ __launch_bounds__(256) void k_gf_far(const G* __restrict__ g0, const G* __restrict__ g1,
                                                 const G* __restrict__ g2, const G* __restrict__ g3, int ng,
                                                 int d2, int nin, int nout, const double* __restrict__ h0,
-                                                const double* __restrict__ h1, int nt, int B, int Mb,
-                                                double* __restrict__ part) {
-    constexpr int NW = KIND == 2 ? 2 : 1;
-    constexpr int WIN = GF_KTILE + GF_M - 1;
-    __shared__ double win[NW][WIN];
-    __shared__ double red[4][2 * GF_M];
-    const int s = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
-    const int r0 = c * GF_RC, r1 = min(r0 + GF_RC, nin);
+                                                const double* __restrict__ h1, int nt, int B, int Mb, int rc,
+                                                int nchunk, double* __restrict__ part) {
+    const int n_og = (nout + 15) >> 4;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= n_og * nchunk) return;
+    const int og = w % n_og, c = w / n_og;
+    const int l = threadIdx.x & 63, q = l >> 4, n = l & 15;
+    const int s = og * 16 + n;
+    const bool s_ok = s < nout;
+    const int r0 = c * rc, r1 = min(r0 + rc, nin);
     const int kmax = min(ng - 1, B + Mb - 1);
-    double acc0[GF_M], acc1[GF_M];
-#pragma unroll
-    for (int m = 0; m < GF_M; ++m) acc0[m] = acc1[m] = 0.0;
+    // U steps of 16 lags per iteration: all their loads are issued before
+    // their MFMAs, so one memory round trip covers 16*U lags.
+    constexpr int U = KIND == 2 ? 2 : 4;
+    gf_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
     for (int r = r0; r < r1; ++r) {
-        const size_t gb = gf_base(KIND, s, r, ng, d2);
-        for (int k0 = 0; k0 <= kmax; k0 += GF_KTILE) {
-            // win[i] = hist[r][B - k0 - (GF_KTILE-1) + i] at times in [0, B)
-            const int tw = B - k0 - (GF_KTILE - 1);
-            __syncthreads();
-            for (int i = tid; i < WIN; i += 256) {
-                const int tt = tw + i;
-                const bool in = tt >= 0 && tt < B;
-                win[0][i] = in ? h0[(size_t)r * nt + tt] : 0.0;
-                if constexpr (NW == 2) win[NW - 1][i] = in ? h1[(size_t)r * nt + tt] : 0.0;
-            }
-            __syncthreads();
-            // lags k0 + KT*tid + j need win[base + 3 - j + m], base = KTILE-KT - KT*tid
-            const int wb = GF_KTILE - GF_KT - GF_KT * tid;
-            double wp[GF_M + GF_KT - 1], wv[NW == 2 ? GF_M + GF_KT - 1 : 1];
+        const size_t gb = gf_base(KIND, s_ok ? s : 0, r, ng, d2);
+        const double* hp = h0 + (size_t)r * nt;
+        const double* hv = KIND == 2 ? h1 + (size_t)r * nt : hp;
+        for (int k0 = 0; k0 <= kmax; k0 += 16 * U) {
+            double ap[U][4], av[U][4];
+            G b0[U][4], b1[U][4], b2[U][4], b3[U][4];
 #pragma unroll
-            for (int q = 0; q < GF_M + GF_KT - 1; ++q) {
-                wp[q] = win[0][wb + q];
-                if constexpr (NW == 2) wv[q] = win[NW - 1][wb + q];
-            }
+            for (int u = 0; u < U; ++u) {
 #pragma unroll
-            for (int j = 0; j < GF_KT; ++j) {
-                const int k = k0 + GF_KT * tid + j;
-                const bool ok = k <= kmax;
-                const double a = ok ? (double)g0[gb + k] : 0.0, b = ok ? (double)g1[gb + k] : 0.0;
-                if constexpr (KIND == 2) {
-                    const double e = ok ? (double)g2[gb + k] : 0.0, f = ok ? (double)g3[gb + k] : 0.0;
-#pragma unroll
-                    for (int m = 0; m < GF_M; ++m) {
-                        const double xp = wp[GF_KT - 1 - j + m], xv = wv[GF_KT - 1 - j + m];
-                        acc0[m] += b * xp + a * xv;  // pv*p + pp*vn
-                        acc1[m] += f * xp + e * xv;  // vv*p + vp*vn
+                for (int j = 0; j < 4; ++j) {
+                    // Loads are unconditional (a guarded load becomes a branch
+                    // with its own wait): lag and time are clamped into the
+                    // buffers and the terms outside the sum are zeroed after.
+                    const int k = k0 + 16 * u + 4 * q + j;
+                    const int tt = B + n - k;  // history time of A[m = n][kk = q]
+                    const bool in = tt >= 0 && tt < B;
+                    const bool kok = s_ok && k <= kmax;
+                    const size_t kc = gb + min(k, kmax);
+                    const int tc = min(max(tt, 0), nt - 1);
+                    const double xp = hp[tc];
+                    const G y0 = g0[kc], y1 = g1[kc];
+                    ap[u][j] = in ? xp : 0.0;
+                    b0[u][j] = kok ? y0 : G(0);
+                    b1[u][j] = kok ? y1 : G(0);
+                    if constexpr (KIND == 2) {
+                        const double xv = hv[tc];
+                        const G y2 = g2[kc], y3 = g3[kc];
+                        av[u][j] = in ? xv : 0.0;
+                        b2[u][j] = kok ? y2 : G(0);
+                        b3[u][j] = kok ? y3 : G(0);
                     }
-                } else {
+                }
+            }
 #pragma unroll
-                    for (int m = 0; m < GF_M; ++m) {
-                        const double x = wp[GF_KT - 1 - j + m];
-                        acc0[m] += a * x;
-                        acc1[m] += b * x;
+            for (int u = 0; u < U; ++u) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if constexpr (KIND == 2) {
+                        acc0 = mfma_f64(ap[u][j], (double)b1[u][j], acc0);  // pv*p
+                        acc0 = mfma_f64(av[u][j], (double)b0[u][j], acc0);  // + pp*vn
+                        acc1 = mfma_f64(ap[u][j], (double)b3[u][j], acc1);  // vv*p
+                        acc1 = mfma_f64(av[u][j], (double)b2[u][j], acc1);  // + vp*vn
+                    } else {
+                        acc0 = mfma_f64(ap[u][j], (double)b0[u][j], acc0);
+                        acc1 = mfma_f64(ap[u][j], (double)b1[u][j], acc1);
                     }
                 }
             }
         }
     }
-    // block sums of the 2*GF_M accumulators
-    const int w = tid >> 6, l = tid & 63;
+    if (!s_ok) return;
 #pragma unroll
-    for (int m = 0; m < GF_M; ++m) {
-        const double s0 = wave_sum(acc0[m]), s1 = wave_sum(acc1[m]);
-        if (l == 0) {
-            red[w][2 * m] = s0;
-            red[w][2 * m + 1] = s1;
+    for (int v = 0; v < 4; ++v) {
+        const int m = 4 * v + q;
+        if (m < Mb) {
+            part[((size_t)(c * GF_M + m) * 2) * nout + s] = acc0[v];
+            part[((size_t)(c * GF_M + m) * 2 + 1) * nout + s] = acc1[v];
         }
-    }
-    __syncthreads();
-    if (tid < 2 * GF_M) {
-        const int m = tid >> 1, ch = tid & 1;
-        const double v = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
-        if (m < Mb) part[((size_t)(c * GF_M + m) * 2 + ch) * nout + s] = v;
     }
 }
 
+// Sum of the far pass's chunk partials in chunk order, into chunk 0's slots:
+// part[(m * 2 + ch) * nout + s], m < Mb.
+static __global__ __launch_bounds__(256) void k_gf_far_sum(double* __restrict__ part, int nout, int Mb, int nchunk) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Mb * 2 * nout) return;
+    const size_t stride = (size_t)GF_M * 2 * nout;
+    double a = part[i];
+    int c = 1;
+    for (; c + 4 <= nchunk; c += 4) {
+        const double x0 = part[i + c * stride], x1 = part[i + (c + 1) * stride];
+        const double x2 = part[i + (c + 2) * stride], x3 = part[i + (c + 3) * stride];
+        a += x0;
+        a += x1;
+        a += x2;
+        a += x3;
+    }
+    for (; c < nchunk; ++c) a += part[i + c * stride];
+    part[i] = a;
+}
+
 // Near pass at step t = B + m: lags 0..min(m, t, ng-1) of every input, plus
-// the far partial sums of the nchunk input chunks, written into the source
-// slots at t+1.
-template <typename T, typename G>
-__global__ __launch_bounds__(256) void k_gf_near(Txs<T> X, int kind, int writeback, const G* __restrict__ g0,
+// the far sums, written into the source slots at t+1.  1024 threads per
+// output; lane l of each 16-lane group takes lag l%16 of one input.  g0..g3
+// are the maps' first GF_M lags laid out [s][r][GF_M], so a workgroup reads
+// one contiguous block of them.
+static_assert(GF_M <= 16, "the near pass has 16 lanes per input, one per lag of the block");
+template <int KIND, typename T, typename G>
+__global__ __launch_bounds__(1024) void k_gf_near(Txs<T> X, int writeback, const G* __restrict__ g0,
                                                  const G* __restrict__ g1, const G* __restrict__ g2,
-                                                 const G* __restrict__ g3, int ng, int d2,
+                                                 const G* __restrict__ g3, int ng,
                                                  const int32_t* __restrict__ iSrc, int nout, int nin,
                                                  const double* __restrict__ h0, const double* __restrict__ h1,
-                                                 double z0, int t, int m, int nchunk,
-                                                 const double* __restrict__ part) {
+                                                 double z0, int t, int m, const double* __restrict__ part) {
     const int s = blockIdx.x, tid = threadIdx.x;
     const int kn = min(m, min(t, ng - 1));
+    const int k = tid & 15;
     double acc0 = 0.0, acc1 = 0.0;
-    for (int r = tid; r < nin; r += 256) {
-        const size_t gb = gf_base(kind, s, r, ng, d2);
-        const double* hr0 = h0 + (size_t)r * X.nt + t;
-        for (int k = 0; k <= kn; ++k) {
-            const double x0 = hr0[-k];
-            if (kind == 2) {
-                const double x1 = h1[(size_t)r * X.nt + t - k];
-                acc0 += (double)g1[gb + k] * x0 + (double)g0[gb + k] * x1;  // pv*p + pp*vn
-                acc1 += (double)g3[gb + k] * x0 + (double)g2[gb + k] * x1;  // vv*p + vp*vn
+    if (k <= kn) {
+#pragma unroll 4
+        for (int r = tid >> 4; r < nin; r += 64) {
+            const size_t gb = ((size_t)s * nin + r) * GF_M + k;
+            const size_t hb = (size_t)r * X.nt + (t - k);
+            const double x0 = h0[hb];
+            if constexpr (KIND == 2) {
+                const double x1 = h1[hb];
+                acc0 += (double)g1[gb] * x0 + (double)g0[gb] * x1;  // pv*p + pp*vn
+                acc1 += (double)g3[gb] * x0 + (double)g2[gb] * x1;  // vv*p + vp*vn
             } else {
-                acc0 += (double)g0[gb + k] * x0;
-                acc1 += (double)g1[gb + k] * x0;
+                acc0 += (double)g0[gb] * x0;
+                acc1 += (double)g1[gb] * x0;
             }
         }
     }
-    __shared__ double red[2][4];
+    __shared__ double red[2][16];
     const double s0 = wave_sum(acc0), s1 = wave_sum(acc1);
     if ((tid & 63) == 0) {
         red[0][tid >> 6] = s0;
@@ -504,14 +551,15 @@ __global__ __launch_bounds__(256) void k_gf_near(Txs<T> X, int kind, int writeba
     }
     __syncthreads();
     if (tid == 0 && writeback && t + 1 < X.nt) {
-        double a0 = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
-        double a1 = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
-        for (int c = 0; c < nchunk; ++c) {
-            a0 += part[((size_t)(c * GF_M + m) * 2) * nout + s];
-            a1 += part[((size_t)(c * GF_M + m) * 2 + 1) * nout + s];
+        double a0 = 0.0, a1 = 0.0;
+        for (int w = 0; w < 16; ++w) {
+            a0 += red[0][w];
+            a1 += red[1][w];
         }
+        a0 += part[(size_t)(m * 2) * nout + s];
+        a1 += part[(size_t)(m * 2 + 1) * nout + s];
         const size_t o = (size_t)iSrc[s] * X.nt + t + 1;
-        if (kind == 1 || kind == 2) {
+        if constexpr (KIND == 1 || KIND == 2) {
             // p_src <- vp-derived, vn_src <- pp-derived (extrapolation.jl:165-167,215-216)
             X.src[o] = (T)a1;
             X.vn_src[o] = (T)a0;

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <tuple>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -120,7 +121,11 @@ static void free_txs(adf_solver* s) {
 }
 
 static void free_gf(adf_solver* s) {
-    for (int i = 0; i < 4; ++i) { dfree(s->gf_t[i]); s->gf_t[i] = nullptr; }
+    for (int i = 0; i < 4; ++i) {
+        dfree(s->gf_t[i]);
+        dfree(s->gf_tn[i]);
+        s->gf_t[i] = s->gf_tn[i] = nullptr;
+    }
     dfree(s->gf_irec); dfree(s->gf_isrc); dfree(s->gf_h0); dfree(s->gf_h1); dfree(s->gf_part);
     s->gf_irec = s->gf_isrc = nullptr;
     s->gf_h0 = s->gf_h1 = s->gf_part = nullptr;
@@ -406,15 +411,32 @@ int solver_set_gf(adf_solver* s, int kind, const double* const* tensors, int64_t
     const int nten = kind == 2 ? 4 : 2;
     const size_t n = (size_t)nt_gf * dim2 * dim3;
     int rc;
+    // The near pass's copy of lags 0..GF_M-1: [s][r][GF_M], zero past nt_gf.
+    const size_t nn = (size_t)n_isrc * n_irec * GF_M;
+    auto near_copy = [&](auto* g, void* dst) -> int {
+        using E = std::remove_const_t<std::remove_pointer_t<decltype(g)>>;
+        std::vector<E> c(nn, E(0));
+        for (int64_t is = 0; is < n_isrc; ++is)
+            for (int64_t ir = 0; ir < n_irec; ++ir) {
+                const size_t b = kind == 1 ? (size_t)nt_gf * (is + dim2 * ir) : (size_t)nt_gf * (ir + dim2 * is);
+                for (int64_t k = 0; k < std::min<int64_t>(nt_gf, GF_M); ++k)
+                    c[((size_t)is * n_irec + ir) * GF_M + k] = g[b + k];
+            }
+        HIPCHK(hipMemcpy(dst, c.data(), nn * sizeof(E), hipMemcpyHostToDevice));
+        return ADF_OK;
+    };
     for (int t = 0; t < nten; ++t) {
         if (!tensors[t]) return fail(ADF_EINVAL, "null GF tensor");
         if ((rc = dmalloc(&s->gf_t[t], n * s->es, "GF tensor"))) return rc;
+        if ((rc = dmalloc(&s->gf_tn[t], nn * s->es, "GF tensor (near lags)"))) return rc;
         if (s->es == 8) {
             HIPCHK(hipMemcpy(s->gf_t[t], tensors[t], n * 8, hipMemcpyHostToDevice));
+            if ((rc = near_copy(tensors[t], s->gf_tn[t]))) return rc;
         } else {
             std::vector<float> f(n);
             to_type(tensors[t], f.data(), n);
             HIPCHK(hipMemcpy(s->gf_t[t], f.data(), n * 4, hipMemcpyHostToDevice));
+            if ((rc = near_copy(f.data(), s->gf_tn[t]))) return rc;
         }
     }
     std::vector<int32_t> ir(n_irec), isr(n_isrc);
@@ -441,7 +463,8 @@ int solver_set_gf(adf_solver* s, int kind, const double* const* tensors, int64_t
     s->gf_d3 = dim3;
     s->gf_nr = n_irec;
     s->gf_ns = n_isrc;
-    s->gf_nchunk = (int)((n_irec + GF_RC - 1) / GF_RC);
+    s->gf_rc = gf_far_rc(n_irec, n_isrc);
+    s->gf_nchunk = (int)((n_irec + s->gf_rc - 1) / s->gf_rc);
     s->gf_isrc_h.assign(iSrc, iSrc + n_isrc);
     s->gf_max_index = 0;
     for (int64_t i = 0; i < n_irec; ++i) s->gf_max_index = std::max(s->gf_max_index, iRec[i]);
@@ -614,23 +637,35 @@ static void launch_gf_gather(adf_solver* s, const Txs<T>& X, int t, hipStream_t 
     const T* g[4] = {(const T*)s->gf_t[0], (const T*)s->gf_t[1], (const T*)s->gf_t[2], (const T*)s->gf_t[3]};
     if (!(s->gf_B >= 0 && t >= s->gf_B && t < s->gf_Bend)) {
         const int64_t B = t, Bend = std::min<int64_t>((t / GF_M + 1) * (int64_t)GF_M, nt - 1);
-        const dim3 grid((unsigned)nout, (unsigned)s->gf_nchunk);
-        const int Mb = (int)(Bend - B);
+        const int Mb = (int)(Bend - B), nchunk = s->gf_nchunk, rc = s->gf_rc;
+        const int waves = (nout + 15) / 16 * nchunk;
+        const dim3 grid((unsigned)((waves + 3) / 4));
         if (s->gf_kind == 2)
             hipLaunchKernelGGL((k_gf_far<2, T>), grid, dim3(256), 0, st, g[0], g[1], g[2], g[3], ng, d2, nin, nout,
-                               s->gf_h0, s->gf_h1, nt, (int)B, Mb, s->gf_part);
+                               s->gf_h0, s->gf_h1, nt, (int)B, Mb, rc, nchunk, s->gf_part);
         else if (s->gf_kind == 1)
             hipLaunchKernelGGL((k_gf_far<1, T>), grid, dim3(256), 0, st, g[0], g[1], g[2], g[3], ng, d2, nin, nout,
-                               s->gf_h0, s->gf_h1, nt, (int)B, Mb, s->gf_part);
+                               s->gf_h0, s->gf_h1, nt, (int)B, Mb, rc, nchunk, s->gf_part);
         else
             hipLaunchKernelGGL((k_gf_far<3, T>), grid, dim3(256), 0, st, g[0], g[1], g[2], g[3], ng, d2, nin, nout,
-                               s->gf_h0, s->gf_h1, nt, (int)B, Mb, s->gf_part);
+                               s->gf_h0, s->gf_h1, nt, (int)B, Mb, rc, nchunk, s->gf_part);
+        hipLaunchKernelGGL(k_gf_far_sum, dim3((unsigned)((Mb * 2 * nout + 255) / 256)), dim3(256), 0, st, s->gf_part,
+                           nout, Mb, nchunk);
         s->gf_B = B;
         s->gf_Bend = Bend;
     }
-    hipLaunchKernelGGL((k_gf_near<T, T>), dim3((unsigned)nout), dim3(256), 0, st, X, s->gf_kind, s->gf_wb, g[0], g[1],
-                       g[2], g[3], ng, d2, s->gf_isrc, nout, nin, s->gf_h0, s->gf_h1, s->dom.z0, t,
-                       (int)(t - s->gf_B), s->gf_nchunk, (const double*)s->gf_part);
+    const int m = (int)(t - s->gf_B);
+    const double* part = s->gf_part;
+    const T* gn[4] = {(const T*)s->gf_tn[0], (const T*)s->gf_tn[1], (const T*)s->gf_tn[2], (const T*)s->gf_tn[3]};
+    if (s->gf_kind == 2)
+        hipLaunchKernelGGL((k_gf_near<2, T, T>), dim3((unsigned)nout), dim3(1024), 0, st, X, s->gf_wb, gn[0], gn[1],
+                           gn[2], gn[3], ng, s->gf_isrc, nout, nin, s->gf_h0, s->gf_h1, s->dom.z0, t, m, part);
+    else if (s->gf_kind == 1)
+        hipLaunchKernelGGL((k_gf_near<1, T, T>), dim3((unsigned)nout), dim3(1024), 0, st, X, s->gf_wb, gn[0], gn[1],
+                           gn[2], gn[3], ng, s->gf_isrc, nout, nin, s->gf_h0, s->gf_h1, s->dom.z0, t, m, part);
+    else
+        hipLaunchKernelGGL((k_gf_near<3, T, T>), dim3((unsigned)nout), dim3(1024), 0, st, X, s->gf_wb, gn[0], gn[1],
+                           gn[2], gn[3], ng, s->gf_isrc, nout, nin, s->gf_h0, s->gf_h1, s->dom.z0, t, m, part);
 }
 
 template <typename T>
