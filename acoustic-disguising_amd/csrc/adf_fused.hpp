@@ -34,6 +34,10 @@ namespace adf {
 // the interior routine, else two.
 constexpr int SWEEP_TY = 2;
 __host__ __device__ constexpr int sweep_ty_side(int cpl) { return cpl >= 4 ? 1 : 2; }
+// Interior rows per tile for es-byte words at cpl cells per lane: SWEEP_TY,
+// except Float64 at 4 cells per lane, where a 2-row tile of the pipelined
+// routine needs more than a wave's 512 registers and spilled to scratch.
+__host__ __device__ constexpr int sweep_ty_int(int es, int cpl) { return es == 8 && cpl >= 4 ? 1 : SWEEP_TY; }
 
 // Kernel arguments of one solve, kept small so the fused kernel does not
 // run out of scalar registers: one base pointer per buffer, offsets derived

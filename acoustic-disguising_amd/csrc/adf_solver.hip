@@ -599,7 +599,7 @@ bool fused_supported(const adf_solver* s) { return fused_cpl(s) > 0; }
 // and even work per CU matter more than the halo plane each tile reloads)
 // and KBG planes per side tile.
 void pick_tiles(const adf_solver* s, int* ty, int* kb, int* kbg) {
-    int t = SWEEP_TY;
+    const int t = sweep_ty_int((int)s->es, fused_cpl(s));
     int k = s->block_z;
     if (!k) {
         const int64_t ny = s->dom.ny, nz = s->dom.nz;
@@ -1134,7 +1134,7 @@ static int batch_build(adf_batch* b) {
     for (adf_solver* s : b->members) cells += s->n3;
     for (int m = 0; m < n; ++m) {
         adf_solver* s = b->members[m];
-        int ty = SWEEP_TY;
+        const int ty = sweep_ty_int((int)s->es, fused_cpl(s));
         // auto: 8-plane interior tiles and 4-plane general tiles, halved
         // while the whole batch has fewer than 8 tiles per CU
         int kb = s->block_z;

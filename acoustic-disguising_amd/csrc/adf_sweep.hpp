@@ -498,7 +498,7 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
     }
 }
 
-// Tile kinds: interior tiles (regular, outside every y/z slab; SWEEP_TY
+// Tile kinds: interior tiles (regular, outside every y/z slab; sweep_ty_int
 // rows), regular tiles in the y and/or z CPML slabs, and general tiles
 // (boundary rows and planes, clamped rows); the last three are side tiles
 // (sweep_ty_side(CPL) rows).  Entries carry their kind in bits 28-30 of the x
@@ -515,7 +515,7 @@ __device__ __forceinline__ void sweep_tile(const Fused<T>& F, uint2 e) {
     const int j0 = (int)(e.x & 0xffffu), k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0xffffu);
     const bool masked = (e.x >> 31) != 0;
     const int kind = (int)((e.x >> 28) & 7u);
-    constexpr int TI = SWEEP_TY, TS = sweep_ty_side(CPL);
+    constexpr int TI = sweep_ty_int((int)sizeof(T), CPL), TS = sweep_ty_side(CPL);
     if (kind == TILE_INT) {
         if constexpr (PF) sweep_tile_pf<T, EXACT, CPL, TI, false, false, POL>(F, j0, k0, k1, masked);
         else sweep_int_tile<T, EXACT, CPL, TI>(F, j0, k0, k1, masked);

@@ -7,15 +7,12 @@
 
 namespace adf {
 
-// The pipelined routines at 2 or 4 cells per lane in Float64 need more
-// registers than a wave has and would spill to scratch memory; the
-// compiler's spill code stores 128-bit registers to scratch and reloads
-// the same registers at once, which has corrupted spilled values here (the
-// store's data is read after the next instruction has overwritten it).
-// Those sizes always take the plane-at-a-time routines, which do not
-// spill; no kernel of this library spills to scratch.
-template <typename T, int CPL>
-constexpr bool pf_fits() { return !(sizeof(T) == 8 && CPL >= 2); }
+// No kernel of this library may spill to scratch memory (the Makefile
+// fails the build if one does): the compiler's spill code stores 128-bit
+// registers to scratch and reloads the same registers at once, which has
+// corrupted spilled values (the store's data is read after the next
+// instruction has overwritten it).  The pipelined routines fit in Float64
+// at 4 cells per lane only with 1-row interior tiles (sweep_ty_int).
 
 // The tiles of list LIST on stream st.
 template <typename T, bool EXACT, int CPL, int LIST>
@@ -25,15 +22,13 @@ static void launch_list(adf_solver* s, const Fused<T>& F, hipStream_t st) {
     const int W = s->waves ? s->waves : 1;
     const dim3 g((unsigned)((n + W - 1) / W));
     const uint2* tl = (const uint2*)s->tiles[LIST];
-    if constexpr (pf_fits<T, CPL>()) {
-        if (s->pipeline && s->store_nt) {
-            hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, ST_NT>), g, dim3(64 * W), 0, st, F, tl, n);
-            return;
-        }
-        if (s->pipeline) {
-            hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, 0>), g, dim3(64 * W), 0, st, F, tl, n);
-            return;
-        }
+    if (s->pipeline && s->store_nt) {
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, ST_NT>), g, dim3(64 * W), 0, st, F, tl, n);
+        return;
+    }
+    if (s->pipeline) {
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, 0>), g, dim3(64 * W), 0, st, F, tl, n);
+        return;
     }
     hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, false, 0>), g, dim3(64 * W), 0, st, F, tl, n);
 }
@@ -75,12 +70,10 @@ static void batch_launch_list(adf_batch* b, const adf_batch::Group& g, hipStream
     const Fused<T>* fs = (const Fused<T>*)b->fs;
     const int pers = b->persistent ? 1 : 0;
     const dim3 blk(64 * g.waves);
-    if constexpr (pf_fits<T, CPL>()) {
-        if (g.pf) {
-            hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, true>), dim3(g.grid[LIST]), blk, 0, st, fs, b->cur,
-                               g.tiles[LIST], g.n[LIST], g.ctr[LIST], pers);
-            return;
-        }
+    if (g.pf) {
+        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, true>), dim3(g.grid[LIST]), blk, 0, st, fs, b->cur,
+                           g.tiles[LIST], g.n[LIST], g.ctr[LIST], pers);
+        return;
     }
     hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, false>), dim3(g.grid[LIST]), blk, 0, st, fs, b->cur,
                        g.tiles[LIST], g.n[LIST], g.ctr[LIST], pers);

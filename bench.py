@@ -17,6 +17,7 @@ restatement of the reference) timed on the host's cores as the CPU baseline.
 Modes for profiling (see profiles/):
   --profile-steps N   run N steps only (no CPU leg), for rocprofv3
   --profile-batch N   run the 64 x 128^3 batched leg for N steps only
+  --profile-gf        run the in-loop KH gather leg only
   --calibrate-copy    time a plain 1 GiB device copy (PMC scale factor)
 """
 from __future__ import annotations
@@ -485,6 +486,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=0)
     ap.add_argument("--profile-batch", type=int, default=0)
+    ap.add_argument("--profile-gf", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-served", action="store_true")
@@ -498,6 +500,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.profile_gf:
+        print(json.dumps(gf_map_leg()))
         return
     if args.profile_batch:
         print(json.dumps(batched_leg(args.batch_solves, [128], args.profile_batch)))
