@@ -102,7 +102,7 @@ struct adf_solver {
     int gf_kind = 0;
     int gf_wb = 1;
     int64_t gf_ng = 0, gf_d2 = 0, gf_d3 = 0;
-    void* gf_t[4] = {nullptr, nullptr, nullptr, nullptr};
+    void* gf_t[4] = {nullptr, nullptr, nullptr, nullptr};   // far-pass tile-major copies (gf_tile_index)
     void* gf_tn[4] = {nullptr, nullptr, nullptr, nullptr};  // lags 0..GF_M-1 as [s][r][GF_M]
     int32_t* gf_irec = nullptr;
     int32_t* gf_isrc = nullptr;
@@ -117,6 +117,9 @@ struct adf_solver {
     int gf_nchunk = 0;
     int64_t gf_max_index = 0;          // largest 1-based iRec/iSrc of the map
     std::vector<int64_t> gf_isrc_h;    // the map's iSrc (1-based), host copy
+    std::vector<int64_t> gf_irec_h;    // the map's iRec (1-based), host copy
+    int32_t* gf_slot_first = nullptr;  // per transceiver: first input slot reading it (-1: none)
+    int32_t* gf_slot_next = nullptr;   // per input slot: next slot reading the same transceiver
     int64_t gf_B = -1, gf_Bend = -1, gf_next = -1;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

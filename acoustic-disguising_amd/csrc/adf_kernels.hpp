@@ -294,21 +294,40 @@ __global__ __launch_bounds__(256) void k_tx_p(T* P, int nx, int ny, int nz, Txs<
     tx_corners<T>(P, X.idx + 3 * is, nx, ny, nz, X.w + 8 * is, X.rec + o, X.src + o, (e >> 31) != 0, live, q, g0);
 }
 
+// The history of a GF map's inputs, [n_irec][nt] doubles, which k_tx_v
+// writes once both records of a step exist:
+// PVFromPin / PFromPin: pin = (p_rec - vn_rec*z0)/2 (extrapolation.jl:224,241)
+// PVFromPV: h0 = p_rec, h1 = vn_rec (extrapolation.jl:207-208).
+// first[is]: the first input slot reading transceiver is (-1: none), next[r]:
+// the next slot reading the same transceiver; first is null without a map
+// with write-back (nothing reads the history then).
+struct GfHist {
+    const int32_t* first;
+    const int32_t* next;
+    double* h0;
+    double* h1;
+    int kind;
+    double z0;
+};
+
 // Velocity transceivers of one level (forward.jl:111-115), then v2vn!
-// (forward.jl:120) and, when no GF map is active, vn2v! into it+1
-// (forward.jl:130-132) for the same transceiver.  32 lanes per
+// (forward.jl:120) and vn2v! into it+1 (forward.jl:130-132) for the same
+// transceiver, and the GF history of the step.  Entries with bit 30 set
+// are outputs of a GF map with write-back: their vn2v! runs after the
+// extrapolation writes their sources (k_gf_near).  32 lanes per
 // transceiver: channel x, y, z on lanes 0-7, 8-15, 16-23.
+constexpr uint32_t TX_INJ = 1u << 31, TX_GF_OUT = 1u << 30, TX_INDEX = TX_GF_OUT - 1;
 template <typename T>
 __global__ __launch_bounds__(256) void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, Txs<T> X,
-                                              const uint32_t* order, int n, int it0, int do_vn2v) {
+                                              const uint32_t* order, int n, int it0, GfHist H) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int g = t >> 5, ch = (t >> 3) & 3, q = t & 7;
     const int lw = (int)(threadIdx.x & 63), g32 = lw & ~31, g0 = lw & ~7;
     const bool any = g < n;
     const bool live = any && ch < 3;
     const uint32_t e = any ? order[g] : 0u;
-    const int is = (int)(e & 0x7fffffffu);
-    const bool inj = (e >> 31) != 0;
+    const int is = (int)(e & TX_INDEX);
+    const bool inj = (e & TX_INJ) != 0, gf_out = (e & TX_GF_OUT) != 0;
     const int ntx = X.ntx, nt = X.nt;
     const size_t o = (size_t)is * nt + it0;
     const size_t cs = (size_t)ntx * nt;  // channel stride of src/rec
@@ -319,45 +338,29 @@ __global__ __launch_bounds__(256) void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int n
     const T rx = __shfl(r, g32, 64), ry = __shfl(r, g32 + 8, 64), rz = __shfl(r, g32 + 16, 64);
     if (!any) return;
     const T* d = X.dir + 4 * is;
-    if (lw == g32) X.vn_rec[o] = rx * d[1] + ry * d[2] + rz * d[3];
-    if (do_vn2v && inj && it0 + 1 < nt && live && q == 0) {
+    if (lw == g32) {
+        const T vnr = rx * d[1] + ry * d[2] + rz * d[3];
+        X.vn_rec[o] = vnr;
+        if (H.first) {
+            const double p = (double)X.rec[o], vn = (double)vnr;
+            for (int r = H.first[is]; r >= 0; r = H.next[r]) {
+                const size_t hb = (size_t)r * nt + it0;
+                if (H.kind == 2) {
+                    H.h0[hb] = p;
+                    H.h1[hb] = vn;
+                } else {
+                    H.h0[hb] = (p - vn * H.z0) / 2.0;
+                }
+            }
+        }
+    }
+    if (inj && !gf_out && it0 + 1 < nt && live && q == 0) {
         const T vn = X.vn_src[o + 1];
         X.src[o + 1 + cs * c1] = X.src[o + 1 + cs * c1] + vn * d[c1];
     }
 }
 
-// vn2v! for every transceiver (used after the GF extrapolation).
-template <typename T>
-__global__ void k_vn2v(Txs<T> X, int it0) {
-    const int is = blockIdx.x * blockDim.x + threadIdx.x;
-    if (is >= X.ntx || it0 + 1 >= X.nt) return;
-    const size_t o = (size_t)is * X.nt + it0 + 1;
-    const size_t cs = (size_t)X.ntx * X.nt;
-    const T vn = X.vn_src[o];
-    const T* d = X.dir + 4 * is;
-#pragma unroll
-    for (int ch = 1; ch <= 3; ++ch) X.src[o + cs * ch] = X.src[o + cs * ch] + vn * d[ch];
-}
-
 // ------------------------------------------------------ GF extrapolation
-// History of the extrapolation inputs, [n_irec][nt] doubles:
-// PVFromPin / PFromPin: pin = (p_rec - vn_rec*z0)/2 (extrapolation.jl:224,241)
-// PVFromPV: h0 = p_rec, h1 = vn_rec (extrapolation.jl:207-208).
-template <typename T>
-__global__ void k_gf_history(Txs<T> X, const int32_t* iRec, int nr, int kind, double z0,
-                             double* h0, double* h1, int it0) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nr) return;
-    const size_t o = (size_t)iRec[r] * X.nt + it0;
-    const double p = (double)X.rec[o], vn = (double)X.vn_rec[o];
-    if (kind == 2) {
-        h0[(size_t)r * X.nt + it0] = p;
-        h1[(size_t)r * X.nt + it0] = vn;
-    } else {
-        h0[(size_t)r * X.nt + it0] = (p - vn * z0) / 2.0;
-    }
-}
-
 // The gather of extrapolation.jl:101-192: for each output s at step t
 // (0-based; written into the source slots at t+1)
 //   out(t)[s] = sum_r sum_{k=0}^{min(t, ng-1)} G[k, s, r] * hist[r][t-k]
@@ -384,9 +387,9 @@ inline int gf_far_rc(int64_t nin, int64_t nout) {
     return (int)std::max<int64_t>(1, (nin * n_og + GF_FAR_WAVES - 1) / GF_FAR_WAVES);
 }
 
-// Element offset of G[0, s, r]: PVFromPin (ng, n_isrc, n_irec); PVFromPV and
-// PFromPin (ng, n_irec, n_isrc).
-__device__ __forceinline__ size_t gf_base(int kind, int s, int r, int ng, int d2) {
+// Element offset of G[0, s, r] in the map as given: PVFromPin (ng, n_isrc,
+// n_irec); PVFromPV and PFromPin (ng, n_irec, n_isrc).
+inline size_t gf_base(int kind, int64_t s, int64_t r, int64_t ng, int64_t d2) {
     return kind == 1 ? (size_t)ng * ((size_t)s + (size_t)d2 * r) : (size_t)ng * ((size_t)r + (size_t)d2 * s);
 }
 
@@ -402,63 +405,79 @@ __device__ __forceinline__ gf_d4 mfma_f64(double a, double b, gf_d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// The far pass reads the map tile-major: for input r, output group og
+// (outputs 16*og..16*og+15) and 16-lag block kb, the 256 words
+//   tile[((r * n_og + og) * nkb + kb) * 256 + kk * 64 + n * 4 + j]
+//     = G[lag 16*kb + 4*kk + j, output 16*og + n, input r]
+// (zero past nt_gf and past the last output), so lane (kk, n) finds its four
+// B operands of a 16-lag step in one aligned 16-byte run and a wave reads
+// 256 consecutive words.
+__host__ __device__ constexpr size_t gf_tile_index(int64_t r, int64_t og, int64_t kb, int n_og, int nkb, int kk,
+                                                   int n, int j) {
+    return (((size_t)r * n_og + og) * nkb + kb) * 256 + kk * 64 + n * 4 + j;
+}
+
 // Far pass of the block [B, B+Mb).  One wave per (16 outputs, chunk of rc
 // inputs): wave w takes output group w % n_og and chunk w / n_og.  Lane l
 // holds A[m = l%16][kk = l/16] and B[kk = l/16][n = l%16]; instruction j of
-// a 16-lag step k0 puts lag k0 + 4*kk + j at K index kk (A and B agree, so
+// a 16-lag step puts lag 16*kb + 4*kk + j at K index kk (A and B agree, so
 // the sum is over all 16 lags).  The accumulator holds D[4v + l/16][l%16] in
 // element v.  Writes part[((c * GF_M + m) * 2 + ch) * nout + s] for m < Mb.
 // No LDS and no barriers: each wave runs on its own.
 template <int KIND, typename G>
 __global__ __launch_bounds__(256) void k_gf_far(const G* __restrict__ g0, const G* __restrict__ g1,
                                                 const G* __restrict__ g2, const G* __restrict__ g3, int ng,
-                                                int d2, int nin, int nout, const double* __restrict__ h0,
+                                                int nin, int nout, const double* __restrict__ h0,
                                                 const double* __restrict__ h1, int nt, int B, int Mb, int rc,
                                                 int nchunk, double* __restrict__ part) {
-    const int n_og = (nout + 15) >> 4;
+    typedef G G4 __attribute__((ext_vector_type(4)));
+    const int n_og = (nout + 15) >> 4, nkb = (ng + 15) >> 4;
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= n_og * nchunk) return;
     const int og = w % n_og, c = w / n_og;
     const int l = threadIdx.x & 63, q = l >> 4, n = l & 15;
     const int s = og * 16 + n;
-    const bool s_ok = s < nout;
     const int r0 = c * rc, r1 = min(r0 + rc, nin);
     const int kmax = min(ng - 1, B + Mb - 1);
+    const int nkbu = kmax / 16 + 1;  // blocks holding lags 0..kmax
     // U steps of 16 lags per iteration: all their loads are issued before
-    // their MFMAs, so one memory round trip covers 16*U lags.
+    // their MFMAs, so one memory round trip covers 16*U lags.  Loads are
+    // unconditional (a guarded load becomes a branch with its own wait):
+    // block and time are clamped into the buffers and the terms outside the
+    // sum are zeroed after.
     constexpr int U = KIND == 2 ? 2 : 4;
     gf_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
     for (int r = r0; r < r1; ++r) {
-        const size_t gb = gf_base(KIND, s_ok ? s : 0, r, ng, d2);
         const double* hp = h0 + (size_t)r * nt;
         const double* hv = KIND == 2 ? h1 + (size_t)r * nt : hp;
-        for (int k0 = 0; k0 <= kmax; k0 += 16 * U) {
+        const size_t tb = gf_tile_index(r, og, 0, n_og, nkb, q, n, 0);
+        for (int kb0 = 0; kb0 < nkbu; kb0 += U) {
             double ap[U][4], av[U][4];
-            G b0[U][4], b1[U][4], b2[U][4], b3[U][4];
+            G4 b0[U], b1[U], b2[U], b3[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
+                const int kb = kb0 + u;
+                const size_t o = tb + (size_t)min(kb, nkbu - 1) * 256;
+                const bool kok = kb < nkbu;
+                const G4 z = {G(0), G(0), G(0), G(0)};
+                const G4 y0 = *(const G4*)(g0 + o), y1 = *(const G4*)(g1 + o);
+                b0[u] = kok ? y0 : z;
+                b1[u] = kok ? y1 : z;
+                if constexpr (KIND == 2) {
+                    const G4 y2 = *(const G4*)(g2 + o), y3 = *(const G4*)(g3 + o);
+                    b2[u] = kok ? y2 : z;
+                    b3[u] = kok ? y3 : z;
+                }
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    // Loads are unconditional (a guarded load becomes a branch
-                    // with its own wait): lag and time are clamped into the
-                    // buffers and the terms outside the sum are zeroed after.
-                    const int k = k0 + 16 * u + 4 * q + j;
-                    const int tt = B + n - k;  // history time of A[m = n][kk = q]
-                    const bool in = tt >= 0 && tt < B;
-                    const bool kok = s_ok && k <= kmax;
-                    const size_t kc = gb + min(k, kmax);
+                    const int tt = B + n - (16 * kb + 4 * q + j);  // history time of A[m = n][kk = q]
+                    const bool in = kok && tt >= 0 && tt < B;
                     const int tc = min(max(tt, 0), nt - 1);
                     const double xp = hp[tc];
-                    const G y0 = g0[kc], y1 = g1[kc];
                     ap[u][j] = in ? xp : 0.0;
-                    b0[u][j] = kok ? y0 : G(0);
-                    b1[u][j] = kok ? y1 : G(0);
                     if constexpr (KIND == 2) {
                         const double xv = hv[tc];
-                        const G y2 = g2[kc], y3 = g3[kc];
                         av[u][j] = in ? xv : 0.0;
-                        b2[u][j] = kok ? y2 : G(0);
-                        b3[u][j] = kok ? y3 : G(0);
                     }
                 }
             }
@@ -479,7 +498,7 @@ __global__ __launch_bounds__(256) void k_gf_far(const G* __restrict__ g0, const 
             }
         }
     }
-    if (!s_ok) return;
+    if (s >= nout) return;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int m = 4 * v + q;
@@ -511,18 +530,19 @@ static __global__ __launch_bounds__(256) void k_gf_far_sum(double* __restrict__ 
 }
 
 // Near pass at step t = B + m: lags 0..min(m, t, ng-1) of every input, plus
-// the far sums, written into the source slots at t+1.  1024 threads per
-// output; lane l of each 16-lane group takes lag l%16 of one input.  g0..g3
-// are the maps' first GF_M lags laid out [s][r][GF_M], so a workgroup reads
-// one contiguous block of them.
+// the far sums, written into the source slots at t+1, then vn2v! for those
+// transceivers (forward.jl:130-132; the others' runs in k_tx_v).  1024
+// threads per output; lane l of each 16-lane group takes lag l%16 of one
+// input.  g0..g3 are the maps' first GF_M lags laid out [s][r][GF_M], so a
+// workgroup reads one contiguous block of them.
 static_assert(GF_M <= 16, "the near pass has 16 lanes per input, one per lag of the block");
 template <int KIND, typename T, typename G>
-__global__ __launch_bounds__(1024) void k_gf_near(Txs<T> X, int writeback, const G* __restrict__ g0,
-                                                 const G* __restrict__ g1, const G* __restrict__ g2,
-                                                 const G* __restrict__ g3, int ng,
-                                                 const int32_t* __restrict__ iSrc, int nout, int nin,
-                                                 const double* __restrict__ h0, const double* __restrict__ h1,
-                                                 double z0, int t, int m, const double* __restrict__ part) {
+__global__ __launch_bounds__(1024) void k_gf_near(Txs<T> X, const G* __restrict__ g0, const G* __restrict__ g1,
+                                                  const G* __restrict__ g2, const G* __restrict__ g3, int ng,
+                                                  const int32_t* __restrict__ iSrc, int nout, int nin,
+                                                  const double* __restrict__ h0, const double* __restrict__ h1,
+                                                  double z0, int t, int m,
+                                                  const double* __restrict__ part) {
     const int s = blockIdx.x, tid = threadIdx.x;
     const int kn = min(m, min(t, ng - 1));
     const int k = tid & 15;
@@ -532,9 +552,8 @@ __global__ __launch_bounds__(1024) void k_gf_near(Txs<T> X, int writeback, const
         for (int r = tid >> 4; r < nin; r += 64) {
             const size_t gb = ((size_t)s * nin + r) * GF_M + k;
             const size_t hb = (size_t)r * X.nt + (t - k);
-            const double x0 = h0[hb];
+            const double x0 = h0[hb], x1 = KIND == 2 ? h1[hb] : 0.0;
             if constexpr (KIND == 2) {
-                const double x1 = h1[hb];
                 acc0 += (double)g1[gb] * x0 + (double)g0[gb] * x1;  // pv*p + pp*vn
                 acc1 += (double)g3[gb] * x0 + (double)g2[gb] * x1;  // vv*p + vp*vn
             } else {
@@ -550,7 +569,7 @@ __global__ __launch_bounds__(1024) void k_gf_near(Txs<T> X, int writeback, const
         red[1][tid >> 6] = s1;
     }
     __syncthreads();
-    if (tid == 0 && writeback && t + 1 < X.nt) {
+    if (tid == 0 && t + 1 < X.nt) {
         double a0 = 0.0, a1 = 0.0;
         for (int w = 0; w < 16; ++w) {
             a0 += red[0][w];
@@ -558,16 +577,23 @@ __global__ __launch_bounds__(1024) void k_gf_near(Txs<T> X, int writeback, const
         }
         a0 += part[(size_t)(m * 2) * nout + s];
         a1 += part[(size_t)(m * 2 + 1) * nout + s];
-        const size_t o = (size_t)iSrc[s] * X.nt + t + 1;
+        const int is = iSrc[s];
+        const size_t o = (size_t)is * X.nt + t + 1;
+        T vn;
         if constexpr (KIND == 1 || KIND == 2) {
             // p_src <- vp-derived, vn_src <- pp-derived (extrapolation.jl:165-167,215-216)
             X.src[o] = (T)a1;
-            X.vn_src[o] = (T)a0;
+            vn = (T)a0;
         } else {
             // PFromPin: p_src = pos - neg, vn_src = (pos + neg)/z0 (extrapolation.jl:246-247)
             X.src[o] = (T)(a0 - a1);
-            X.vn_src[o] = (T)((a0 + a1) / z0);
+            vn = (T)((a0 + a1) / z0);
         }
+        X.vn_src[o] = vn;
+        const size_t cs = (size_t)X.ntx * X.nt;
+        const T* d = X.dir + 4 * is;
+#pragma unroll
+        for (int ch = 1; ch <= 3; ++ch) X.src[o + cs * ch] = X.src[o + cs * ch] + vn * d[ch];
     }
 }
 

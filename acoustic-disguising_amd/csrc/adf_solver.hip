@@ -111,6 +111,8 @@ static void dfree(void* p) {
 static void free_txs(adf_solver* s) {
     dfree(s->tx_idx); dfree(s->tx_w); dfree(s->tx_src); dfree(s->tx_rec); dfree(s->tx_dir);
     dfree(s->tx_vn_src); dfree(s->tx_vn_rec); dfree(s->order_p); dfree(s->order_v); dfree(s->fix_cells);
+    dfree(s->gf_slot_first); dfree(s->gf_slot_next);
+    s->gf_slot_first = s->gf_slot_next = nullptr;
     s->fix_cells = nullptr;
     s->n_fix = 0;
     s->tx_idx = nullptr; s->tx_w = s->tx_src = s->tx_rec = s->tx_dir = s->tx_vn_src = s->tx_vn_rec = nullptr;
@@ -133,6 +135,7 @@ static void free_gf(adf_solver* s) {
     s->gf_B = s->gf_Bend = s->gf_next = -1;
     s->gf_max_index = 0;
     s->gf_isrc_h.clear();
+    s->gf_irec_h.clear();
 }
 
 void solver_free(adf_solver* s) {
@@ -224,7 +227,7 @@ static void make_levels(int64_t ntx, int nx, int ny, int nz, const std::vector<i
     order.assign((size_t)ntx, 0);
     std::vector<int> pos(off.begin(), off.end() - 1);
     for (int64_t is = 0; is < ntx; ++is)
-        order[pos[lev[is] - 1]++] = (uint32_t)is | (inj[is] ? 0x80000000u : 0u);
+        order[pos[lev[is] - 1]++] = (uint32_t)is | (inj[is] ? TX_INJ : 0u);
 }
 
 // Injector flags from host source data: p channel from src[0]; velocity
@@ -267,6 +270,30 @@ int rebuild_schedule(adf_solver* s) {
     std::vector<uint32_t> order_v;
     const int cv[3] = {1, 2, 3};
     make_levels(ntx, nx, ny, nz, s->h_idx, iv, cv, 3, order_v, s->lev_v);
+    // GF map with write-back: its outputs' vn2v! runs after the
+    // extrapolation, and k_tx_v writes its inputs' history (first/next
+    // input slot per transceiver)
+    dfree(s->gf_slot_first);
+    dfree(s->gf_slot_next);
+    s->gf_slot_first = s->gf_slot_next = nullptr;
+    if (s->gf_kind && s->gf_wb) {
+        std::vector<uint8_t> out(ntx, 0);
+        for (int64_t i : s->gf_isrc_h) out[i - 1] = 1;
+        for (uint32_t& e : order_v)
+            if (out[e & TX_INDEX]) e |= TX_GF_OUT;
+        const int64_t nr = (int64_t)s->gf_irec_h.size();
+        std::vector<int32_t> first(ntx, -1), next(nr, -1);
+        for (int64_t r = nr - 1; r >= 0; --r) {
+            const int64_t is = s->gf_irec_h[r] - 1;
+            next[r] = first[is];
+            first[is] = (int32_t)r;
+        }
+        int rc = dmalloc((void**)&s->gf_slot_first, (size_t)ntx * 4, "GF input slots");
+        if (!rc) rc = dmalloc((void**)&s->gf_slot_next, (size_t)nr * 4, "GF input slots");
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(s->gf_slot_first, first.data(), (size_t)ntx * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(s->gf_slot_next, next.data(), (size_t)nr * 4, hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpyAsync(s->order_v, order_v.data(), (size_t)ntx * 4, hipMemcpyHostToDevice, st));
 
     // Velocity cells whose update reads a pressure cell an injector writes:
@@ -396,7 +423,13 @@ int solver_set_gf(adf_solver* s, int kind, const double* const* tensors, int64_t
                   int64_t dim3, const int64_t* iRec, int64_t n_irec, const int64_t* iSrc, int64_t n_isrc,
                   int writeback) {
     hipStreamSynchronize(s->ctx->stream);
+    // a previous map with write-back flagged its outputs in the schedule
+    const bool had_wb = s->gf_kind && s->gf_wb;
     free_gf(s);
+    if (had_wb) {
+        const int rc = rebuild_schedule(s);
+        if (rc) return rc;
+    }
     if (kind == 0) return ADF_OK;
     if (kind < 1 || kind > 3 || !tensors || nt_gf < 1 || !iRec || !iSrc)
         return fail(ADF_EINVAL, "adf_solver_set_gf_map: bad kind or null arrays");
@@ -408,35 +441,56 @@ int solver_set_gf(adf_solver* s, int kind, const double* const* tensors, int64_t
         if (iRec[i] < 1 || iRec[i] > s->ntx) return fail(ADF_EINVAL, "iRec out of transceiver range");
     for (int64_t i = 0; i < n_isrc; ++i)
         if (iSrc[i] < 1 || iSrc[i] > s->ntx) return fail(ADF_EINVAL, "iSrc out of transceiver range");
+    if (writeback) {
+        // each output owns its transceiver's source slots (the outputs are
+        // written in parallel, then converted by vn2v! once)
+        std::vector<uint8_t> seen(s->ntx, 0);
+        for (int64_t i = 0; i < n_isrc; ++i) {
+            if (seen[iSrc[i] - 1])
+                return fail(ADF_EINVAL, "adf_solver_set_gf_map: iSrc names transceiver " + std::to_string(iSrc[i]) +
+                                            " twice; with write-back every output needs its own transceiver");
+            seen[iSrc[i] - 1] = 1;
+        }
+    }
     const int nten = kind == 2 ? 4 : 2;
     const size_t n = (size_t)nt_gf * dim2 * dim3;
     int rc;
-    // The near pass's copy of lags 0..GF_M-1: [s][r][GF_M], zero past nt_gf.
-    const size_t nn = (size_t)n_isrc * n_irec * GF_M;
-    auto near_copy = [&](auto* g, void* dst) -> int {
+    // Two device copies of each tensor in the storage type: the far pass's
+    // tile-major map (gf_tile_index; zero past nt_gf and the last output)
+    // and the near pass's lags 0..GF_M-1 as [s][r][GF_M] (zero past nt_gf).
+    const int n_og = (int)((n_isrc + 15) / 16), nkb = (int)((nt_gf + 15) / 16);
+    const size_t ntile = (size_t)n_irec * n_og * nkb * 256, nn = (size_t)n_isrc * n_irec * GF_M;
+    auto upload = [&](auto* g, void* dtile, void* dnear) -> int {
         using E = std::remove_const_t<std::remove_pointer_t<decltype(g)>>;
-        std::vector<E> c(nn, E(0));
+        std::vector<E> c(ntile, E(0));
+        for (int64_t ir = 0; ir < n_irec; ++ir)
+            for (int64_t is = 0; is < n_isrc; ++is) {
+                const E* gs = g + gf_base(kind, is, ir, nt_gf, dim2);
+                for (int64_t k = 0; k < nt_gf; ++k)
+                    c[gf_tile_index(ir, is / 16, k / 16, n_og, nkb, (int)(k % 16) / 4, (int)(is % 16), (int)(k % 4))] =
+                        gs[k];
+            }
+        HIPCHK(hipMemcpy(dtile, c.data(), ntile * sizeof(E), hipMemcpyHostToDevice));
+        std::vector<E> d(nn, E(0));
         for (int64_t is = 0; is < n_isrc; ++is)
             for (int64_t ir = 0; ir < n_irec; ++ir) {
-                const size_t b = kind == 1 ? (size_t)nt_gf * (is + dim2 * ir) : (size_t)nt_gf * (ir + dim2 * is);
+                const E* gs = g + gf_base(kind, is, ir, nt_gf, dim2);
                 for (int64_t k = 0; k < std::min<int64_t>(nt_gf, GF_M); ++k)
-                    c[((size_t)is * n_irec + ir) * GF_M + k] = g[b + k];
+                    d[((size_t)is * n_irec + ir) * GF_M + k] = gs[k];
             }
-        HIPCHK(hipMemcpy(dst, c.data(), nn * sizeof(E), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dnear, d.data(), nn * sizeof(E), hipMemcpyHostToDevice));
         return ADF_OK;
     };
     for (int t = 0; t < nten; ++t) {
         if (!tensors[t]) return fail(ADF_EINVAL, "null GF tensor");
-        if ((rc = dmalloc(&s->gf_t[t], n * s->es, "GF tensor"))) return rc;
+        if ((rc = dmalloc(&s->gf_t[t], ntile * s->es, "GF tensor (far-pass tiles)"))) return rc;
         if ((rc = dmalloc(&s->gf_tn[t], nn * s->es, "GF tensor (near lags)"))) return rc;
         if (s->es == 8) {
-            HIPCHK(hipMemcpy(s->gf_t[t], tensors[t], n * 8, hipMemcpyHostToDevice));
-            if ((rc = near_copy(tensors[t], s->gf_tn[t]))) return rc;
+            if ((rc = upload(tensors[t], s->gf_t[t], s->gf_tn[t]))) return rc;
         } else {
             std::vector<float> f(n);
             to_type(tensors[t], f.data(), n);
-            HIPCHK(hipMemcpy(s->gf_t[t], f.data(), n * 4, hipMemcpyHostToDevice));
-            if ((rc = near_copy(f.data(), s->gf_tn[t]))) return rc;
+            if ((rc = upload(f.data(), s->gf_t[t], s->gf_tn[t]))) return rc;
         }
     }
     std::vector<int32_t> ir(n_irec), isr(n_isrc);
@@ -454,10 +508,8 @@ int solver_set_gf(adf_solver* s, int kind, const double* const* tensors, int64_t
     }
     s->gf_kind = kind;
     s->gf_wb = writeback ? 1 : 0;
-    if (writeback) {
+    if (writeback)
         for (int64_t i = 0; i < n_isrc; ++i) s->gf_inj[iSrc[i] - 1] = 1;
-        if ((rc = rebuild_schedule(s))) return rc;
-    }
     s->gf_ng = nt_gf;
     s->gf_d2 = dim2;
     s->gf_d3 = dim3;
@@ -466,13 +518,14 @@ int solver_set_gf(adf_solver* s, int kind, const double* const* tensors, int64_t
     s->gf_rc = gf_far_rc(n_irec, n_isrc);
     s->gf_nchunk = (int)((n_irec + s->gf_rc - 1) / s->gf_rc);
     s->gf_isrc_h.assign(iSrc, iSrc + n_isrc);
+    s->gf_irec_h.assign(iRec, iRec + n_irec);
     s->gf_max_index = 0;
     for (int64_t i = 0; i < n_irec; ++i) s->gf_max_index = std::max(s->gf_max_index, iRec[i]);
     for (int64_t i = 0; i < n_isrc; ++i) s->gf_max_index = std::max(s->gf_max_index, iSrc[i]);
     if ((rc = dmalloc((void**)&s->gf_part, (size_t)s->gf_nchunk * GF_M * 2 * n_isrc * 8, "GF partial sums")))
         return rc;
     s->gf_B = s->gf_Bend = s->gf_next = -1;
-    return ADF_OK;
+    return writeback ? rebuild_schedule(s) : ADF_OK;
 }
 
 void* cur_field(adf_solver* s) { return s->fieldb[s->cur]; }
@@ -632,7 +685,7 @@ static void launch_tx_p(adf_solver* s, const Solve<T>& S, const Txs<T>& X, int i
 // of a run that does not continue the previous one), then the near pass.
 template <typename T>
 static void launch_gf_gather(adf_solver* s, const Txs<T>& X, int t, hipStream_t st) {
-    const int nt = (int)s->dom.nt, ng = (int)s->gf_ng, d2 = (int)s->gf_d2;
+    const int nt = (int)s->dom.nt, ng = (int)s->gf_ng;
     const int nin = (int)s->gf_nr, nout = (int)s->gf_ns;
     const T* g[4] = {(const T*)s->gf_t[0], (const T*)s->gf_t[1], (const T*)s->gf_t[2], (const T*)s->gf_t[3]};
     if (!(s->gf_B >= 0 && t >= s->gf_B && t < s->gf_Bend)) {
@@ -641,13 +694,13 @@ static void launch_gf_gather(adf_solver* s, const Txs<T>& X, int t, hipStream_t 
         const int waves = (nout + 15) / 16 * nchunk;
         const dim3 grid((unsigned)((waves + 3) / 4));
         if (s->gf_kind == 2)
-            hipLaunchKernelGGL((k_gf_far<2, T>), grid, dim3(256), 0, st, g[0], g[1], g[2], g[3], ng, d2, nin, nout,
+            hipLaunchKernelGGL((k_gf_far<2, T>), grid, dim3(256), 0, st, g[0], g[1], g[2], g[3], ng, nin, nout,
                                s->gf_h0, s->gf_h1, nt, (int)B, Mb, rc, nchunk, s->gf_part);
         else if (s->gf_kind == 1)
-            hipLaunchKernelGGL((k_gf_far<1, T>), grid, dim3(256), 0, st, g[0], g[1], g[2], g[3], ng, d2, nin, nout,
+            hipLaunchKernelGGL((k_gf_far<1, T>), grid, dim3(256), 0, st, g[0], g[1], g[2], g[3], ng, nin, nout,
                                s->gf_h0, s->gf_h1, nt, (int)B, Mb, rc, nchunk, s->gf_part);
         else
-            hipLaunchKernelGGL((k_gf_far<3, T>), grid, dim3(256), 0, st, g[0], g[1], g[2], g[3], ng, d2, nin, nout,
+            hipLaunchKernelGGL((k_gf_far<3, T>), grid, dim3(256), 0, st, g[0], g[1], g[2], g[3], ng, nin, nout,
                                s->gf_h0, s->gf_h1, nt, (int)B, Mb, rc, nchunk, s->gf_part);
         hipLaunchKernelGGL(k_gf_far_sum, dim3((unsigned)((Mb * 2 * nout + 255) / 256)), dim3(256), 0, st, s->gf_part,
                            nout, Mb, nchunk);
@@ -658,14 +711,14 @@ static void launch_gf_gather(adf_solver* s, const Txs<T>& X, int t, hipStream_t 
     const double* part = s->gf_part;
     const T* gn[4] = {(const T*)s->gf_tn[0], (const T*)s->gf_tn[1], (const T*)s->gf_tn[2], (const T*)s->gf_tn[3]};
     if (s->gf_kind == 2)
-        hipLaunchKernelGGL((k_gf_near<2, T, T>), dim3((unsigned)nout), dim3(1024), 0, st, X, s->gf_wb, gn[0], gn[1],
-                           gn[2], gn[3], ng, s->gf_isrc, nout, nin, s->gf_h0, s->gf_h1, s->dom.z0, t, m, part);
+        hipLaunchKernelGGL((k_gf_near<2, T, T>), dim3((unsigned)nout), dim3(1024), 0, st, X, gn[0], gn[1], gn[2],
+                           gn[3], ng, s->gf_isrc, nout, nin, s->gf_h0, s->gf_h1, s->dom.z0, t, m, part);
     else if (s->gf_kind == 1)
-        hipLaunchKernelGGL((k_gf_near<1, T, T>), dim3((unsigned)nout), dim3(1024), 0, st, X, s->gf_wb, gn[0], gn[1],
-                           gn[2], gn[3], ng, s->gf_isrc, nout, nin, s->gf_h0, s->gf_h1, s->dom.z0, t, m, part);
+        hipLaunchKernelGGL((k_gf_near<1, T, T>), dim3((unsigned)nout), dim3(1024), 0, st, X, gn[0], gn[1], gn[2],
+                           gn[3], ng, s->gf_isrc, nout, nin, s->gf_h0, s->gf_h1, s->dom.z0, t, m, part);
     else
-        hipLaunchKernelGGL((k_gf_near<3, T, T>), dim3((unsigned)nout), dim3(1024), 0, st, X, s->gf_wb, gn[0], gn[1],
-                           gn[2], gn[3], ng, s->gf_isrc, nout, nin, s->gf_h0, s->gf_h1, s->dom.z0, t, m, part);
+        hipLaunchKernelGGL((k_gf_near<3, T, T>), dim3((unsigned)nout), dim3(1024), 0, st, X, gn[0], gn[1], gn[2],
+                           gn[3], ng, s->gf_isrc, nout, nin, s->gf_h0, s->gf_h1, s->dom.z0, t, m, part);
 }
 
 template <typename T>
@@ -673,22 +726,16 @@ static void launch_after_v(adf_solver* s, const Solve<T>& S, const Txs<T>& X, in
     const int it0 = (int)(it - 1);
     const int nt = (int)s->dom.nt;
     if (s->ntx) {
-        const int do_vn2v = s->gf_kind == 0 ? 1 : 0;
+        const GfHist H{s->gf_slot_first, s->gf_slot_next, s->gf_h0, s->gf_h1, s->gf_kind, s->dom.z0};
         for (size_t L = 0; L + 1 < s->lev_v.size(); ++L) {
             const int n = s->lev_v[L + 1] - s->lev_v[L];
             if (n > 0)
                 hipLaunchKernelGGL((k_tx_v<T>), dim3((32 * n + 255) / 256), dim3(256), 0, st, S.Vx, S.Vy, S.Vz, S.nx, S.ny,
-                                   S.nz, X, s->order_v + s->lev_v[L], n, it0, do_vn2v);
+                                   S.nz, X, s->order_v + s->lev_v[L], n, it0, H);
         }
-        if (s->gf_kind) {
-            const int nr = (int)s->gf_nr;
-            hipLaunchKernelGGL((k_gf_history<T>), dim3((nr + 255) / 256), dim3(256), 0, st, X, s->gf_irec, nr,
-                               s->gf_kind, s->dom.z0, s->gf_h0, s->gf_h1, it0);
-            if (it < nt) {
-                if (s->gf_wb) launch_gf_gather<T>(s, X, it0, st);
-                hipLaunchKernelGGL((k_vn2v<T>), dim3((unsigned)((s->ntx + 255) / 256)), dim3(256), 0, st, X, it0);
-            }
-        }
+        // The extrapolation (extrapolation.jl:101-249) and the map outputs'
+        // vn2v!; without write-back nothing reads the map's history.
+        if (s->gf_kind && s->gf_wb && it < nt) launch_gf_gather<T>(s, X, it0, st);
     }
     if (!S.fuse_mirror) {
         const int n = std::max(S.nx * S.nz, S.nx * S.ny);

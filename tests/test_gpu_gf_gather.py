@@ -109,3 +109,44 @@ def test_set_transceivers_keeps_the_map_consistent():
     s.set_transceivers(A.transceivers_to_grid(d, txs, np.float64))
     s.run()
     assert np.array_equal(a, s.get_trace(L.ADF_SRC_P))
+
+
+def _solver(d, cpml, g):
+    s = A.Solver(d, A.dtype_code(np.float64), A.make_cpml(d, cpml.n, cpml.rcoef, cpml.fc, np.float64))
+    s.set_transceivers(g)
+    return s
+
+
+def test_replacing_a_map_leaves_no_stale_outputs():
+    """A transceiver with a vn drive of its own is an output of a first map
+    and not of a second.  After the second map replaces the first (or the
+    first is removed), the run equals one that only ever had the second map
+    (or none), bit for bit: that transceiver's vn2v! runs again with the
+    velocity transceivers instead of after the extrapolation."""
+    d, txs, cpml, mask, iRec, iSrc = _case(18, 50, 8, 6)
+    _, gf1 = _map("pv_from_pin", 8, 6, 11, iRec, iSrc, seed=8)
+    _, gf2 = _map("pv_from_pin", 8, 5, 11, iRec, iSrc[:5], seed=9)
+    g = A.transceivers_to_grid(d, txs, np.float64)
+    g.vn_src[iSrc[5] - 1] = A.ricker_wavelet(d, 2000.0, True) * 1e-3
+
+    def run(maps):
+        s = _solver(d, cpml, g)
+        for m in maps:
+            s.set_gf_map(m)
+        s.run()
+        return [s.get_trace(w) for w in range(10)] + [s.get_field()]
+
+    for got, want in ((run([gf1, gf2]), run([gf2])), (run([gf1, None]), run([]))):
+        assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    assert np.abs(run([gf2])[L.ADF_SRC_VX][iSrc[5] - 1]).max() > 0
+
+
+def test_duplicate_outputs_are_refused_with_write_back():
+    d, txs, cpml, mask, iRec, iSrc = _case(16, 30, 6, 3)
+    dup = [iSrc[0], iSrc[1], iSrc[0]]
+    s = _solver(d, cpml, A.transceivers_to_grid(d, txs, np.float64))
+    with pytest.raises(L.AdfError, match="twice"):
+        s.set_gf_map(_map("pv_from_pin", 6, 3, 9, iRec, dup, seed=2)[1])
+    # without write-back nothing is written, so the map is accepted
+    s.set_gf_map(_map("pv_from_pin", 6, 3, 9, iRec, dup, seed=2, writeback=False)[1])
+    s.run()
