@@ -156,8 +156,11 @@ int adf_solver_set_trace(adf_solver* s, int which, const void* in);
  * as host doubles in Julia layout.  iRec/iSrc are 1-based transceiver
  * columns.  writeback 1 adds the extrapolated sources into the source slots
  * at it+1 (the documented intent, docs/impulsive_gf_hologram_path.md:154-171);
- * 0 computes and discards them, as the reference's copy semantics do
- * (extrapolation.jl:215-216,226-232,246-247).  kind 0 clears the map. */
+ * iSrc must then name distinct transceivers (ADF_EINVAL otherwise: two
+ * outputs would write the same source slots).  0 leaves the source slots as
+ * they are: the reference computes the sources into copies and discards
+ * them (extrapolation.jl:215-216,226-232,246-247), which has no observable
+ * result, so nothing is computed.  kind 0 clears the map. */
 #define ADF_GF_NONE 0
 #define ADF_GF_PV_FROM_PIN 1
 #define ADF_GF_PV_FROM_PV 2
