@@ -18,6 +18,7 @@ Modes for profiling (see profiles/):
   --profile-steps N   run N steps only (no CPU leg), for rocprofv3
   --profile-batch N   run the 64 x 128^3 batched leg for N steps only
   --profile-gf        run the in-loop KH gather leg only
+  --profile-hologram  run the 192^3 hologram synthesis leg only
   --calibrate-copy    time a plain 1 GiB device copy (PMC scale factor)
 """
 from __future__ import annotations
@@ -335,6 +336,49 @@ def reverb_leg(n_ill=32, n=128, nt=1200, n_rec=100):
             "inner_p_peak": float(np.abs(g["inner_p"]).max())}
 
 
+def hologram_leg(n=192, n_kh=2000, n_ff=500, steps=1500):
+    """Hologram synthesis (build_hologram's FDTD, src/kernels/forward.jl
+    with C_for_N injection, illumination.jl): 192^3 FP32, 12-cell CPML, cube
+    of edge 30 cells; every point of a 2000-point KH sphere (r = 40 cells)
+    injects q and f through C_for_N with seeded Ricker-filtered Gaussian
+    noise drives, and 500 far-field receivers (r = 80 cells) record.  Whole
+    steps by wall clock, the sweep by HIP events every 16th step."""
+    import adfdtd as A
+    from adfdtd import _lib as L
+
+    p = 12
+    dx = C_AIR / (2.5 * FC) / 10
+    xm = (n - 1) * dx / 2
+    d0 = A.make_domain(1.0, xm, xm, xm, n, n, n, C_AIR, RHO_AIR, 0.5)
+    d = A.make_domain((steps + 1 - 0.5) * d0.dt, xm, xm, xm, n, n, n, C_AIR, RHO_AIR, 0.5)
+    kh, khn = A.fibonacci_sphere(n_kh, 40 * d.dx)
+    ff, ffn = A.fibonacci_sphere(n_ff, 80 * d.dx)
+    rng = np.random.default_rng(2)
+    rick = A.ricker_wavelet(d, FC, False)
+    drive = lambda: np.convolve(rng.standard_normal(d.nt), rick)[: d.nt]
+    cf = A.c_for_n(n_kh, 40 * d.dx, d)
+    txs = [A.make_transceiver(d, kh[i], [cf["C_q"], 0.0, 0.0, 0.0], drive()) for i in range(n_kh)]
+    txs += [A.make_transceiver(d, kh[i], np.r_[0.0, cf["C_f"] * khn[i]], drive()) for i in range(n_kh)]
+    z = np.zeros(d.nt)
+    txs += [A.make_transceiver(d, ff[i], np.r_[0.0, ffn[i]], z) for i in range(n_ff)]
+    s = A.Solver(d, A.dtype_code(np.float32, "fast"), A.make_cpml(d, p, 1e-4, FC, np.float32))
+    s.build_mask("cube", [30 * d.dx])
+    shape = A.tune_solver(s)
+    s.set_transceivers(A.transceivers_to_grid(d, txs, np.float32))
+    s.set_option(L.ADF_OPT_TIME_SWEEP, SWEEP_SAMPLE)
+    t0 = time.perf_counter()
+    s.run(1, steps)
+    wall = time.perf_counter() - t0
+    sweep_ms, launches = s.sweep_stats()
+    peak = float(np.abs(s.get_trace(L.ADF_REC_P)[2 * n_kh:]).max())
+    b = algorithmic_bytes_per_step(n, p, 4)
+    return {"grid": n, "steps": steps, "injectors": 2 * n_kh, "receivers": n_ff, "wall_s": round(wall, 4),
+            "ms_per_step": round(wall / steps * 1e3, 5), "glups": round(n ** 3 * steps / wall / 1e9, 2),
+            "sweep_ms": round(sweep_ms, 5), "hbm_frac_sweep": round(b / (sweep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "hbm_frac_step": round(b / (wall / steps) / 1e9 / HBM_PEAK_GBS, 4), "tiling": shape,
+            "far_field_p_peak": peak}
+
+
 def gf_map_leg(n=192, nio=400, ng=827, steps=100):
     """In-loop KH gather (extrapolation.jl:101-249) at paper scale: a 192^3
     FP32 solve (12-cell CPML, cube r=30 cells) whose 400 outer receivers
@@ -487,6 +531,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=0)
     ap.add_argument("--profile-batch", type=int, default=0)
     ap.add_argument("--profile-gf", action="store_true")
+    ap.add_argument("--profile-hologram", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-served", action="store_true")
@@ -503,6 +548,9 @@ def main():
         return
     if args.profile_gf:
         print(json.dumps(gf_map_leg()))
+        return
+    if args.profile_hologram:
+        print(json.dumps(hologram_leg()))
         return
     if args.profile_batch:
         print(json.dumps(batched_leg(args.batch_solves, [128], args.profile_batch)))
@@ -577,6 +625,10 @@ def main():
                                      "(npml 0), sphere r=16 cells, 100 outer + 100 inner receivers, 1200 steps, "
                                      "resampled to fs_out = 5 fc", **reverb_leg()}
         log("reverb leg done")
+        res["hologram"] = {"workload": "hologram synthesis: 192^3 FP32, 12-cell CPML, cube edge 30 cells, 2000 KH points "
+                                       "injecting q and f via C_for_N (seeded Ricker-filtered noise drives), 500 "
+                                       "far-field receivers at 80 cells, 1500 steps", **hologram_leg()}
+        log("hologram leg done")
         res["kh_in_loop"] = {"workload": "in-loop KH gather, 192^3 FP32 solve, 400 outer receivers -> 400 inner "
                                          "sources through a PVFromPin map of 827 lags (paper scale)", **gf_map_leg()}
         log("in-loop KH gather leg done")
