@@ -11,6 +11,7 @@ memory p[0] (p+1,ny,nz,2) -> (2,nz,ny,p+1), etc.
 """
 from __future__ import annotations
 
+import functools
 import math
 from dataclasses import dataclass, field as dc_field
 from fractions import Fraction
@@ -69,7 +70,8 @@ def make_domain(tmax=10e-4, xmax=1.0, ymax=1.0, zmax=1.0, nx=201, ny=201, nz=201
 # the single rounding, like `x(d) .- dx/2` on a TwicePrecision range.
 
 
-def _axis(xmax: float, n: int, dx: float, staggered: bool) -> np.ndarray:
+@functools.lru_cache(maxsize=64)
+def _axis_cached(xmax: float, n: int, dx: float, staggered: bool) -> np.ndarray:
     fx = Fraction(xmax)
     h = Fraction(dx) / 2
     out = np.empty(n)
@@ -78,7 +80,14 @@ def _axis(xmax: float, n: int, dx: float, staggered: bool) -> np.ndarray:
         if staggered:
             v -= h
         out[i - 1] = float(v)
+    out.flags.writeable = False
     return out
+
+
+def _axis(xmax: float, n: int, dx: float, staggered: bool) -> np.ndarray:
+    # the exact-rational evaluation costs milliseconds per axis and every
+    # transceivers_to_grid call of a domain needs the same six axes
+    return _axis_cached(float(xmax), int(n), float(dx), bool(staggered)).copy()
 
 
 def x(d: Domain) -> np.ndarray:
