@@ -150,3 +150,25 @@ def test_duplicate_outputs_are_refused_with_write_back():
     # without write-back nothing is written, so the map is accepted
     s.set_gf_map(_map("pv_from_pin", 6, 3, 9, iRec, dup, seed=2, writeback=False)[1])
     s.run()
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_blocked_gather_fp32_within_1e5_of_the_float64_oracle(kind):
+    """The FP32 solve carries the maps as float32 words (sums still in
+    double): extrapolated sources and the traces they drive stay within
+    1e-5 relative L2 of the Float64 oracle, over several 16-step blocks."""
+    d, txs, cpml, mask, iRec, iSrc = _case(22, 90, 40, 7)
+    ref_args, gf = _map(kind, 40, 7, 40, iRec, iSrc, seed=21)
+    ref = oracle_solver(O, d, txs, cpml, mask, "f64")
+    ref.set_gf_map(*ref_args)
+    ref.run()
+    s = A.Solver(d, A.dtype_code(np.float32, "fast"), A.make_cpml(d, cpml.n, cpml.rcoef, cpml.fc, np.float32))
+    s.set_transceivers(A.transceivers_to_grid(d, txs, np.float32))
+    s.set_gf_map(gf)
+    s.run()
+    src = np.stack([s.get_trace(L.ADF_SRC_P), s.get_trace(L.ADF_VN_SRC)])
+    rsrc = np.stack([ref.tx["src"][0], ref.tx["vn_src"]])
+    errs = (rel_l2(src, rsrc), rel_l2(s.get_trace(L.ADF_REC_P), ref.tx["rec"][0]))
+    print(kind, errs, flush=True)
+    assert np.abs(rsrc[:, 8:]).max() > 0
+    assert max(errs) <= 1e-5, errs
