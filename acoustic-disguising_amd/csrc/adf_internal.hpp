@@ -109,9 +109,8 @@ struct adf_solver {
     int64_t gf_nr = 0, gf_ns = 0;
     double* gf_h0 = nullptr;
     double* gf_h1 = nullptr;
-    // blocked gather: far-pass partial sums [chunk][GF_M][2][n_isrc], the
-    // steps [gf_B, gf_Bend) (0-based) they cover, and the step after the
-    // last one run (a run continuing from it keeps the current block)
+    // blocked gather: far-pass partial sums [chunk][GF_M][2][n_isrc] of the
+    // block of GF_M steps starting at step gf_B (0-based; -1: none)
     double* gf_part = nullptr;
     int gf_rc = 1;
     int gf_nchunk = 0;
@@ -120,7 +119,10 @@ struct adf_solver {
     std::vector<int64_t> gf_irec_h;    // the map's iRec (1-based), host copy
     int32_t* gf_slot_first = nullptr;  // per transceiver: first input slot reading it (-1: none)
     int32_t* gf_slot_next = nullptr;   // per input slot: next slot reading the same transceiver
-    int64_t gf_B = -1, gf_Bend = -1, gf_next = -1;
+    int64_t gf_B = -1;
+    // the history (gf_h0/gf_h1) equals the one computed from the record
+    // traces at times [0, gf_hist_valid)
+    int64_t gf_hist_valid = 0;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_step_ms = 0.0;

@@ -361,6 +361,25 @@ __global__ __launch_bounds__(256) void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int n
 }
 
 // ------------------------------------------------------ GF extrapolation
+// The GF history of input r at times [t0, t1) from the record traces, as
+// k_tx_v writes it at each step.
+template <typename T>
+__global__ void k_gf_hist_fill(Txs<T> X, const int32_t* iRec, int nin, int kind, double z0, double* h0, double* h1,
+                               int t0, int t1) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int len = t1 - t0;
+    if (i >= (int64_t)nin * len) return;
+    const int r = (int)(i / len), tt = t0 + (int)(i % len);
+    const size_t o = (size_t)iRec[r] * X.nt + tt, hb = (size_t)r * X.nt + tt;
+    const double p = (double)X.rec[o], vn = (double)X.vn_rec[o];
+    if (kind == 2) {
+        h0[hb] = p;
+        h1[hb] = vn;
+    } else {
+        h0[hb] = (p - vn * z0) / 2.0;
+    }
+}
+
 // The gather of extrapolation.jl:101-192: for each output s at step t
 // (0-based; written into the source slots at t+1)
 //   out(t)[s] = sum_r sum_{k=0}^{min(t, ng-1)} G[k, s, r] * hist[r][t-k]

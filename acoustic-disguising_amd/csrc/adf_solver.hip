@@ -132,7 +132,8 @@ static void free_gf(adf_solver* s) {
     s->gf_irec = s->gf_isrc = nullptr;
     s->gf_h0 = s->gf_h1 = s->gf_part = nullptr;
     s->gf_kind = 0;
-    s->gf_B = s->gf_Bend = s->gf_next = -1;
+    s->gf_B = -1;
+    s->gf_hist_valid = 0;
     s->gf_max_index = 0;
     s->gf_isrc_h.clear();
     s->gf_irec_h.clear();
@@ -155,7 +156,6 @@ void solver_free(adf_solver* s) {
 }
 
 int solver_zero_state(adf_solver* s) {
-    s->gf_B = s->gf_Bend = s->gf_next = -1;  // the blocked gather starts over
     hipStream_t st = s->ctx->stream;
     for (int b = 0; b < 2; ++b) {
         HIPCHK(hipMemsetAsync(s->fieldb[b], 0, 4 * s->n3 * s->es, st));
@@ -390,6 +390,8 @@ int solver_set_txs(adf_solver* s, int64_t ntx, const int64_t* idx, const void* w
     for (int c = 0; c < 3; ++c) s->src_inj_v[c].assign(ntx, 0);
     s->vn_inj.assign(ntx, 0);
     s->gf_inj.assign(ntx, 0);
+    s->gf_hist_valid = 0;  // the record traces start over
+    s->gf_B = -1;
     // a GF map set before keeps writing its sources: they stay injectors
     if (s->gf_kind && s->gf_wb)
         for (int64_t i : s->gf_isrc_h) s->gf_inj[i - 1] = 1;
@@ -403,6 +405,11 @@ int solver_set_txs(adf_solver* s, int64_t ntx, const int64_t* idx, const void* w
 // A source buffer was overwritten from the host (set_trace): refresh the
 // injector flags of that buffer and the schedule.
 int solver_trace_written(adf_solver* s, int which, const void* data) {
+    if (which == 0 || which == 4) {  // p_rec / vn_rec: the GF history is rebuilt from them
+        s->gf_hist_valid = 0;
+        s->gf_B = -1;
+        return ADF_OK;
+    }
     if (which >= 5 && which <= 8) injectors_from(s, which - 5, data);
     else if (which == 9) injectors_from(s, 4, data);
     else return ADF_OK;
@@ -524,7 +531,8 @@ int solver_set_gf(adf_solver* s, int kind, const double* const* tensors, int64_t
     for (int64_t i = 0; i < n_isrc; ++i) s->gf_max_index = std::max(s->gf_max_index, iSrc[i]);
     if ((rc = dmalloc((void**)&s->gf_part, (size_t)s->gf_nchunk * GF_M * 2 * n_isrc * 8, "GF partial sums")))
         return rc;
-    s->gf_B = s->gf_Bend = s->gf_next = -1;
+    s->gf_B = -1;
+    s->gf_hist_valid = 0;
     return writeback ? rebuild_schedule(s) : ADF_OK;
 }
 
@@ -681,15 +689,18 @@ static void launch_tx_p(adf_solver* s, const Solve<T>& S, const Txs<T>& X, int i
 }
 
 // The blocked GF gather of step t (0-based; the history of t is recorded):
-// a far pass when t starts a block (a multiple of GF_M, or the first step
-// of a run that does not continue the previous one), then the near pass.
+// the far pass of t's block (steps B = GF_M * floor(t / GF_M) onward) when
+// the partial sums held are another block's, then the near pass.  Blocks
+// start at multiples of GF_M whatever step a run starts at, so the sums of
+// a step do not depend on how the steps were split into runs.
 template <typename T>
 static void launch_gf_gather(adf_solver* s, const Txs<T>& X, int t, hipStream_t st) {
     const int nt = (int)s->dom.nt, ng = (int)s->gf_ng;
     const int nin = (int)s->gf_nr, nout = (int)s->gf_ns;
     const T* g[4] = {(const T*)s->gf_t[0], (const T*)s->gf_t[1], (const T*)s->gf_t[2], (const T*)s->gf_t[3]};
-    if (!(s->gf_B >= 0 && t >= s->gf_B && t < s->gf_Bend)) {
-        const int64_t B = t, Bend = std::min<int64_t>((t / GF_M + 1) * (int64_t)GF_M, nt - 1);
+    const int64_t B = (int64_t)(t / GF_M) * GF_M;
+    if (s->gf_B != B) {
+        const int64_t Bend = std::min<int64_t>(B + GF_M, nt - 1);
         const int Mb = (int)(Bend - B), nchunk = s->gf_nchunk, rc = s->gf_rc;
         const int waves = (nout + 15) / 16 * nchunk;
         const dim3 grid((unsigned)((waves + 3) / 4));
@@ -705,9 +716,8 @@ static void launch_gf_gather(adf_solver* s, const Txs<T>& X, int t, hipStream_t 
         hipLaunchKernelGGL(k_gf_far_sum, dim3((unsigned)((Mb * 2 * nout + 255) / 256)), dim3(256), 0, st, s->gf_part,
                            nout, Mb, nchunk);
         s->gf_B = B;
-        s->gf_Bend = Bend;
     }
-    const int m = (int)(t - s->gf_B);
+    const int m = (int)(t - B);
     const double* part = s->gf_part;
     const T* gn[4] = {(const T*)s->gf_tn[0], (const T*)s->gf_tn[1], (const T*)s->gf_tn[2], (const T*)s->gf_tn[3]};
     if (s->gf_kind == 2)
@@ -992,7 +1002,17 @@ static int run_impl(adf_solver* s, int64_t start, int64_t end) {
     s->sweep_avg_ms = 0.0;
     s->sweep_n = 0;
     s->sweep_seq = 0;
-    if (start - 1 != s->gf_next) s->gf_B = s->gf_Bend = -1;  // not a continuation: new gather block
+    if (s->gf_kind && s->gf_wb && s->gf_hist_valid < start - 1) {
+        // the extrapolation reads the record traces of every earlier step
+        // (extrapolation.jl:202-249): steps this solver has not run, or
+        // whose traces were replaced, are taken from the traces
+        const int t0 = (int)s->gf_hist_valid, t1 = (int)(start - 1);
+        const int64_t n = s->gf_nr * (int64_t)(t1 - t0);
+        Txs<T> X = make_txs<T>(s);
+        hipLaunchKernelGGL((k_gf_hist_fill<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, X, s->gf_irec,
+                           (int)s->gf_nr, s->gf_kind, s->dom.z0, s->gf_h0, s->gf_h1, t0, t1);
+        s->gf_B = -1;
+    }
     HIPCHK(hipEventRecord(s->ev0, st));
     for (int64_t it = start; it <= end; ++it) {
         step_impl<T, EXACT>(s, it, st, fused, ty);
@@ -1009,7 +1029,7 @@ static int run_impl(adf_solver* s, int64_t start, int64_t end) {
         }
     }
     HIPCHK(hipGetLastError());
-    s->gf_next = end;
+    if (s->gf_kind && s->gf_wb) s->gf_hist_valid = std::max(s->gf_hist_valid, end);
     HIPCHK(hipEventRecord(s->ev1, st));
     HIPCHK(hipEventSynchronize(s->ev1));
     float ms = 0.f;

@@ -172,3 +172,37 @@ def test_blocked_gather_fp32_within_1e5_of_the_float64_oracle(kind):
     print(kind, errs, flush=True)
     assert np.abs(rsrc[:, 8:]).max() > 0
     assert max(errs) <= 1e-5, errs
+
+
+@pytest.mark.parametrize("kind", ["pv_from_pin", "pv_from_pv"])
+def test_resume_in_a_fresh_solver_from_saved_state(kind):
+    """A run resumed at step 41 in a fresh solver, from the field, CPML
+    memory and the ten trace buffers saved after step 40, gives the same
+    bits as one run: the extrapolation takes the history of steps 1-40 from
+    the restored record traces, as the reference reads txs.p.rec and
+    txs.vn_rec (extrapolation.jl:202-249), and its 16-step blocks do not
+    depend on where a run starts."""
+    d, txs, cpml, mask, iRec, iSrc = _case(18, 60, 20, 5)
+    _, gf = _map(kind, 20, 5, 23, iRec, iSrc, seed=3)
+
+    def solver():
+        s = A.Solver(d, A.dtype_code(np.float64), A.make_cpml(d, cpml.n, cpml.rcoef, cpml.fc, np.float64))
+        s.set_transceivers(A.transceivers_to_grid(d, txs, np.float64))
+        s.set_gf_map(gf)
+        return s
+
+    one = solver()
+    one.run(1, d.nt)
+    a = solver()
+    a.run(1, 40)
+    cp = A.make_cpml(d, cpml.n, cpml.rcoef, cpml.fc, np.float64)
+    a.get_cpml_memory(cp)
+    b = solver()
+    b.set_field(a.get_field())
+    b.set_cpml_memory(cp)
+    for w in range(10):
+        b.set_trace(w, a.get_trace(w))
+    b.run(41, d.nt)
+    assert np.abs(one.get_trace(L.ADF_SRC_P)[iSrc[0] - 1]).max() > 0
+    assert all(np.array_equal(one.get_trace(w), b.get_trace(w)) for w in range(10))
+    assert np.array_equal(one.get_field(), b.get_field())
