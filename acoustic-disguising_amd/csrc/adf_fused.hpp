@@ -620,14 +620,11 @@ __device__ __forceinline__ int xcd_chunk_start(int x, int n) { return x * (n >> 
 __device__ __forceinline__ int xcd_chunk_size(int x, int n) { return (n >> 3) + (x < (n & 7) ? 1 : 0); }
 __device__ __forceinline__ int xcd_tile(int b, int n) { return xcd_chunk_start(b & 7, n) + (b >> 3); }
 
-// Velocity cells whose update reads an injected pressure cell (and, for
-// tilings that split rows, seam cells), recomputed from V0 / psi0 and the
-// injected P1 (after k_tx_p).  cells: packed (axis << 30 | cell) entries.
+// One velocity cell whose update reads an injected pressure cell,
+// recomputed from V0 / psi0 and the injected P1 (after the pressure
+// transceivers).  e: packed (axis << 30 | cell) entry.
 template <typename T, bool EXACT>
-__global__ void k_fixup_v(Fused<T> F, const uint32_t* cells, int n) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const uint32_t e = cells[t];
+__device__ __forceinline__ void fixup_cell(const Fused<T>& F, uint32_t e) {
     const int ax = (int)(e >> 30);
     const int c = (int)(e & 0x3fffffffu);
     const int nx = F.nx, ny = F.ny, nz = F.nz;
@@ -645,6 +642,70 @@ __global__ void k_fixup_v(Fused<T> F, const uint32_t* cells, int n) {
         F.V1(2)[c] = v;
         if (k == 1 && F.fuse_mirror) F.V1(2)[c - nx * ny] = -v;
     }
+}
+
+template <typename T, bool EXACT>
+__global__ void k_fixup_v(Fused<T> F, const uint32_t* cells, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) fixup_cell<T, EXACT>(F, cells[t]);
+}
+
+// The transceiver work of one fused step in one launch (forward.jl:92-132;
+// k_tx_p levels, k_fixup_v and k_tx_v levels otherwise).  The chain is the
+// transceivers that inject, or whose stencils touch a cell written by an
+// injection or by the velocity fix-up, with the level offsets of their
+// order_p / order_v subsequences; all other transceivers only record.
+constexpr int TXC_MAXLEV = 8;
+struct TxChain {
+    const uint32_t* chain_p;  // order_p entries of the chain, level by level
+    const uint32_t* chain_v;  // order_v entries of the chain, level by level
+    const uint32_t* fix;      // velocity fix-up cells
+    const uint32_t* rec;      // indices of the record-only transceivers
+    int n_fix, n_rec;
+    int nlp, nlv;
+    int lp[TXC_MAXLEV + 1], lv[TXC_MAXLEV + 1];
+};
+
+// Block 0 runs the chain in the reference order: the pressure levels, the
+// fix-up, the velocity levels, with a workgroup barrier after each (every
+// wave of the block is on one CU, so the barrier also orders the block's
+// global stores and loads).  Blocks 1.. take 8 record-only transceivers
+// each (32 lanes per transceiver): their records read cells nothing in
+// this launch writes, so they run beside the chain.
+template <typename T, bool EXACT>
+__global__ __launch_bounds__(256) void k_tx_step(Fused<T> F, Txs<T> X, TxChain C, int it0, GfHist H) {
+    const int tid = (int)threadIdx.x, lw = tid & 63;
+    T* P = F.P1();
+    T* Vx = F.V1(0);
+    T* Vy = F.V1(1);
+    T* Vz = F.V1(2);
+    if (blockIdx.x == 0) {
+        for (int L = 0; L < C.nlp; ++L) {
+            const int end = C.lp[L + 1];
+            for (int b = C.lp[L]; b < end; b += 32) {
+                const int g = b + (tid >> 3);
+                const bool live = g < end;
+                tx_p_entry<T>(P, F.nx, F.ny, F.nz, X, live ? C.chain_p[g] : 0u, live, lw, it0);
+            }
+            __syncthreads();
+        }
+        for (int b = 0; b < C.n_fix; b += 256)
+            if (b + tid < C.n_fix) fixup_cell<T, EXACT>(F, C.fix[b + tid]);
+        __syncthreads();
+        for (int L = 0; L < C.nlv; ++L) {
+            const int end = C.lv[L + 1];
+            for (int b = C.lv[L]; b < end; b += 8) {
+                const int g = b + (tid >> 5);
+                const bool any = g < end;
+                tx_v_entry<T>(Vx, Vy, Vz, F.nx, F.ny, F.nz, X, any ? C.chain_v[g] : 0u, any, lw, it0, H);
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    const int g = (int)(blockIdx.x - 1) * 8 + (tid >> 5);
+    const bool any = g < C.n_rec;
+    tx_record_entry<T>(P, Vx, Vy, Vz, F.nx, F.ny, F.nz, X, any ? (int)C.rec[g] : 0, any, lw, it0, H);
 }
 
 }  // namespace adf

@@ -79,6 +79,12 @@ struct adf_solver {
     // velocity cells next to pressure-injection stencils (fused path)
     uint32_t* fix_cells = nullptr;
     int n_fix = 0;
+    // one-launch transceiver step (k_tx_step): device lists chain_p |
+    // chain_v | record-only indices, level offsets of the two chains
+    uint32_t* txc_list = nullptr;
+    int txc_np = 0, txc_nv = 0, txc_nrec = 0;
+    std::vector<int> txc_lp, txc_lv;
+    bool txc_ok = false;  // the chain fits k_tx_step's limits
 
     // fused-sweep tile lists for tiling (tl_ty, tl_kb, tl_kbg, tl_w):
     // interior tiles, side tiles (CPML-slab and boundary) and all tiles
@@ -97,6 +103,7 @@ struct adf_solver {
     int pipeline = 1;     // pipelined (prefetching) tile routines
     int store_nt = 0;     // non-temporal stores in the pipelined routines
     int launch = 0;       // ADF_SWEEP_* (one launch, two streams, two serial)
+    int tx_launch = 1;    // ADF_OPT_TX_LAUNCH
 
     // GF extrapolation map
     int gf_kind = 0;

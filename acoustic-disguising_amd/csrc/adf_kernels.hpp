@@ -282,16 +282,22 @@ __device__ __forceinline__ T tx_corners(T* F, const int32_t* id3, int nx, int ny
 // Pressure transceivers of one conflict-free level (forward.jl:92), 8 lanes
 // per transceiver.  order entries carry the injector flag in bit 31:
 // transceivers whose p source is zero for the whole run only record.
+// One order_p entry e over 8 lanes (lw: lane in the wave); all 8 lanes call.
+template <typename T>
+__device__ __forceinline__ void tx_p_entry(T* P, int nx, int ny, int nz, const Txs<T>& X, uint32_t e, bool live,
+                                           int lw, int it0) {
+    const int is = (int)(e & 0x7fffffffu);
+    const size_t o = (size_t)is * X.nt + it0;
+    tx_corners<T>(P, X.idx + 3 * is, nx, ny, nz, X.w + 8 * is, X.rec + o, X.src + o, (e >> 31) != 0, live, lw & 7,
+                  lw & ~7);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_tx_p(T* P, int nx, int ny, int nz, Txs<T> X, const uint32_t* order, int n,
                                               int it0) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int g = t >> 3, q = t & 7, g0 = (int)(threadIdx.x & 63) & ~7;
+    const int g = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 3);
     const bool live = g < n;
-    const uint32_t e = live ? order[g] : 0u;
-    const int is = (int)(e & 0x7fffffffu);
-    const size_t o = (size_t)is * X.nt + it0;
-    tx_corners<T>(P, X.idx + 3 * is, nx, ny, nz, X.w + 8 * is, X.rec + o, X.src + o, (e >> 31) != 0, live, q, g0);
+    tx_p_entry<T>(P, nx, ny, nz, X, live ? order[g] : 0u, live, (int)(threadIdx.x & 63), it0);
 }
 
 // The history of a GF map's inputs, [n_irec][nt] doubles, which k_tx_v
@@ -317,15 +323,28 @@ struct GfHist {
 // extrapolation writes their sources (k_gf_near).  32 lanes per
 // transceiver: channel x, y, z on lanes 0-7, 8-15, 16-23.
 constexpr uint32_t TX_INJ = 1u << 31, TX_GF_OUT = 1u << 30, TX_INDEX = TX_GF_OUT - 1;
+
+// The GF history of step it0 for transceiver is from its records p and vn.
+__device__ __forceinline__ void gf_hist_write(const GfHist& H, int is, int nt, int it0, double p, double vn) {
+    for (int r = H.first[is]; r >= 0; r = H.next[r]) {
+        const size_t hb = (size_t)r * nt + it0;
+        if (H.kind == 2) {
+            H.h0[hb] = p;
+            H.h1[hb] = vn;
+        } else {
+            H.h0[hb] = (p - vn * H.z0) / 2.0;
+        }
+    }
+}
+
+// One order_v entry e over the 32 lanes of a wave half (lw: lane in the
+// wave; channel x, y, z on lanes 0-7, 8-15, 16-23 of the half; `any`: the
+// entry is real).  All 32 lanes must call this (shuffles).
 template <typename T>
-__global__ __launch_bounds__(256) void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, Txs<T> X,
-                                              const uint32_t* order, int n, int it0, GfHist H) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int g = t >> 5, ch = (t >> 3) & 3, q = t & 7;
-    const int lw = (int)(threadIdx.x & 63), g32 = lw & ~31, g0 = lw & ~7;
-    const bool any = g < n;
+__device__ __forceinline__ void tx_v_entry(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, const Txs<T>& X, uint32_t e,
+                                           bool any, int lw, int it0, const GfHist& H) {
+    const int ch = (lw >> 3) & 3, q = lw & 7, g32 = lw & ~31, g0 = lw & ~7;
     const bool live = any && ch < 3;
-    const uint32_t e = any ? order[g] : 0u;
     const int is = (int)(e & TX_INDEX);
     const bool inj = (e & TX_INJ) != 0, gf_out = (e & TX_GF_OUT) != 0;
     const int ntx = X.ntx, nt = X.nt;
@@ -341,23 +360,52 @@ __global__ __launch_bounds__(256) void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int n
     if (lw == g32) {
         const T vnr = rx * d[1] + ry * d[2] + rz * d[3];
         X.vn_rec[o] = vnr;
-        if (H.first) {
-            const double p = (double)X.rec[o], vn = (double)vnr;
-            for (int r = H.first[is]; r >= 0; r = H.next[r]) {
-                const size_t hb = (size_t)r * nt + it0;
-                if (H.kind == 2) {
-                    H.h0[hb] = p;
-                    H.h1[hb] = vn;
-                } else {
-                    H.h0[hb] = (p - vn * H.z0) / 2.0;
-                }
-            }
-        }
+        if (H.first) gf_hist_write(H, is, nt, it0, (double)X.rec[o], (double)vnr);
     }
     if (inj && !gf_out && it0 + 1 < nt && live && q == 0) {
         const T vn = X.vn_src[o + 1];
         X.src[o + 1 + cs * c1] = X.src[o + 1 + cs * c1] + vn * d[c1];
     }
+}
+
+// Velocity transceivers of one level (forward.jl:111-115), then v2vn!
+// (forward.jl:120) and vn2v! into it+1 (forward.jl:130-132) for the same
+// transceiver, and the GF history of the step.  Entries with bit 30 set
+// are outputs of a GF map with write-back: their vn2v! runs after the
+// extrapolation writes their sources (k_gf_near).  32 lanes per
+// transceiver: channel x, y, z on lanes 0-7, 8-15, 16-23.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tx_v(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, Txs<T> X,
+                                              const uint32_t* order, int n, int it0, GfHist H) {
+    const int g = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 5);
+    const bool any = g < n;
+    tx_v_entry<T>(Vx, Vy, Vz, nx, ny, nz, X, any ? order[g] : 0u, any, (int)(threadIdx.x & 63), it0, H);
+}
+
+// Pressure and velocity records of one transceiver that injects nothing
+// and whose stencils touch no cell written by the step's injections or
+// velocity fix-up (so its records do not depend on the transceiver order),
+// over the 32 lanes of a wave half: channel x, y, z on lanes 0-7, 8-15,
+// 16-23 and pressure on lanes 24-31, then v2vn! and the GF history.  The
+// same operations as k_tx_p + k_tx_v for such an entry.
+template <typename T>
+__device__ __forceinline__ void tx_record_entry(T* P, T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, const Txs<T>& X,
+                                                int is, bool any, int lw, int it0, const GfHist& H) {
+    const int ch = (lw >> 3) & 3, q = lw & 7, g32 = lw & ~31, g0 = lw & ~7;
+    const int ntx = X.ntx, nt = X.nt;
+    const size_t o = (size_t)is * nt + it0;
+    const size_t cs = (size_t)ntx * nt;
+    const int c = ch < 3 ? ch + 1 : 0;  // buffer channel: P, Vx, Vy, Vz
+    T* F = c == 0 ? P : (c == 1 ? Vx : (c == 2 ? Vy : Vz));
+    const T r = tx_corners<T>(F, X.idx + 3 * (is + ntx * c), nx, ny, nz, X.w + 8 * (is + ntx * c),
+                              X.rec + o + cs * c, X.src + o + cs * c, false, any, q, g0);
+    const T rx = __shfl(r, g32, 64), ry = __shfl(r, g32 + 8, 64), rz = __shfl(r, g32 + 16, 64);
+    const T rp = __shfl(r, g32 + 24, 64);
+    if (!any || lw != g32) return;
+    const T* d = X.dir + 4 * is;
+    const T vnr = rx * d[1] + ry * d[2] + rz * d[3];
+    X.vn_rec[o] = vnr;
+    if (H.first) gf_hist_write(H, is, nt, it0, (double)rp, (double)vnr);
 }
 
 // ------------------------------------------------------ GF extrapolation

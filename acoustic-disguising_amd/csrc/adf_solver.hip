@@ -9,6 +9,7 @@
 #include <tuple>
 #include <type_traits>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "adf_internal.hpp"
@@ -111,10 +112,12 @@ static void dfree(void* p) {
 static void free_txs(adf_solver* s) {
     dfree(s->tx_idx); dfree(s->tx_w); dfree(s->tx_src); dfree(s->tx_rec); dfree(s->tx_dir);
     dfree(s->tx_vn_src); dfree(s->tx_vn_rec); dfree(s->order_p); dfree(s->order_v); dfree(s->fix_cells);
-    dfree(s->gf_slot_first); dfree(s->gf_slot_next);
+    dfree(s->gf_slot_first); dfree(s->gf_slot_next); dfree(s->txc_list);
     s->gf_slot_first = s->gf_slot_next = nullptr;
     s->fix_cells = nullptr;
     s->n_fix = 0;
+    s->txc_list = nullptr;
+    s->txc_ok = false;
     s->tx_idx = nullptr; s->tx_w = s->tx_src = s->tx_rec = s->tx_dir = s->tx_vn_src = s->tx_vn_rec = nullptr;
     s->order_p = s->order_v = nullptr;
     s->ntx = 0;
@@ -251,6 +254,93 @@ static void injectors_from(adf_solver* s, int which, const void* data) {
     }
 }
 
+// The lists of the one-launch transceiver step (k_tx_step).  The chain is
+// every transceiver that injects (ip/iv), is a GF map output, or has a
+// stencil corner on a cell the step writes after the sweep: pressure cells
+// of the injectors, velocity cells of the fix-up (with their mirror cells)
+// and of the velocity injectors.  It keeps its order_p / order_v levels.
+// The others only record, from cells nothing in the launch writes, so their
+// records do not depend on the order.  Chains with more than TXC_MAXLEV
+// levels or more than TXC_MAX entries keep the per-level launches.
+static constexpr int TXC_MAX = 512, TXC_MAX_FIX = 4096;
+static int build_tx_chain(adf_solver* s, const std::vector<uint8_t>& ip, const std::vector<uint8_t>& iv,
+                          const std::vector<uint32_t>& order_p, const std::vector<uint32_t>& order_v,
+                          const std::vector<uint32_t>& fix) {
+    dfree(s->txc_list);
+    s->txc_list = nullptr;
+    s->txc_ok = false;
+    const int64_t ntx = s->ntx;
+    const int nx = (int)s->dom.nx, ny = (int)s->dom.ny, nz = (int)s->dom.nz;
+    const int64_t n3 = (int64_t)nx * ny * nz;
+    auto corners = [&](int64_t is, int ch, int64_t* out) {
+        const int32_t* id = &s->h_idx[(size_t)3 * (is + ntx * ch)];
+        const int I = id[0], J = id[1], K = id[2];
+        if (!(I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz)) return false;
+        for (int q = 0; q < 8; ++q)
+            out[q] = (int64_t)(I - 1 + (q & 1)) + (int64_t)nx * ((J - 1 + ((q >> 1) & 1)) + (int64_t)ny * (K - 1 + (q >> 2)));
+        return true;
+    };
+    // written cells, keyed channel * n3 + cell (channel 0 = P, 1..3 = Vx..Vz)
+    std::unordered_set<int64_t> written;
+    int64_t cs[8];
+    for (int64_t is = 0; is < ntx; ++is) {
+        if (ip[is] && corners(is, 0, cs))
+            for (int q = 0; q < 8; ++q) written.insert(cs[q]);
+        if (iv[is])
+            for (int ch = 1; ch <= 3; ++ch)
+                if (corners(is, ch, cs))
+                    for (int q = 0; q < 8; ++q) written.insert(ch * n3 + cs[q]);
+    }
+    for (uint32_t e : fix) {
+        const int64_t ax = e >> 30, c = e & 0x3fffffffu;
+        written.insert((ax + 1) * n3 + c);
+        if (ax == 1 && c >= nx) written.insert(2 * n3 + c - nx);                        // Vy mirror
+        if (ax == 2 && c >= (int64_t)nx * ny) written.insert(3 * n3 + c - (int64_t)nx * ny);  // Vz mirror
+    }
+    std::vector<uint8_t> dep((size_t)ntx, 0);
+    for (uint32_t e : order_v)
+        if (e & TX_GF_OUT) dep[e & TX_INDEX] = 1;
+    for (int64_t is = 0; is < ntx; ++is) {
+        if (ip[is] || iv[is]) dep[is] = 1;
+        for (int ch = 0; ch <= 3 && !dep[is]; ++ch)
+            if (corners(is, ch, cs))
+                for (int q = 0; q < 8; ++q)
+                    if (written.count(ch * n3 + cs[q])) { dep[is] = 1; break; }
+    }
+    // the chain's subsequences of the two level orders, empty levels dropped
+    auto chain = [&](const std::vector<uint32_t>& order, const std::vector<int>& lev, std::vector<uint32_t>& out,
+                     std::vector<int>& off) {
+        off.assign(1, 0);
+        for (size_t L = 0; L + 1 < lev.size(); ++L) {
+            for (int x = lev[L]; x < lev[L + 1]; ++x)
+                if (dep[order[x] & TX_INDEX]) out.push_back(order[x]);
+            if ((int)out.size() > off.back()) off.push_back((int)out.size());
+        }
+    };
+    std::vector<uint32_t> cp, cv, rec;
+    chain(order_p, s->lev_p, cp, s->txc_lp);
+    chain(order_v, s->lev_v, cv, s->txc_lv);
+    for (int64_t is = 0; is < ntx; ++is)
+        if (!dep[is]) rec.push_back((uint32_t)is);
+    if ((int)s->txc_lp.size() - 1 > TXC_MAXLEV || (int)s->txc_lv.size() - 1 > TXC_MAXLEV || (int)cp.size() > TXC_MAX ||
+        (int)cv.size() > TXC_MAX || (int)fix.size() > TXC_MAX_FIX)
+        return ADF_OK;
+    s->txc_np = (int)cp.size();
+    s->txc_nv = (int)cv.size();
+    s->txc_nrec = (int)rec.size();
+    std::vector<uint32_t> all;
+    all.insert(all.end(), cp.begin(), cp.end());
+    all.insert(all.end(), cv.begin(), cv.end());
+    all.insert(all.end(), rec.begin(), rec.end());
+    if (!all.empty()) {
+        int rc = dmalloc((void**)&s->txc_list, all.size() * 4, "transceiver chain");
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(s->txc_list, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+    }
+    s->txc_ok = true;
+    return ADF_OK;
+}
+
 // Levels, fixup cells and the mirror decision from the current injector
 // flags; uploads the schedule.
 int rebuild_schedule(adf_solver* s) {
@@ -339,7 +429,7 @@ int rebuild_schedule(adf_solver* s) {
             if (ch == 3 && K <= 2) s->fuse_mirror = false;
         }
     }
-    return ADF_OK;
+    return build_tx_chain(s, ip, iv, order, order_v, cells);
 }
 
 int solver_set_txs(adf_solver* s, int64_t ntx, const int64_t* idx, const void* w, const void* src,
@@ -573,6 +663,9 @@ int solver_set_option(adf_solver* s, int option, int64_t value) {
         if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_PIPELINE must be 0 or 1");
         s->pipeline = (int)value;
         touch(s);
+    } else if (option == ADF_OPT_TX_LAUNCH) {
+        if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_TX_LAUNCH must be 0 or 1");
+        s->tx_launch = (int)value;
     } else if (option == ADF_OPT_SWEEP_LAUNCH) {
         if (value < ADF_SWEEP_ONE_LAUNCH || value > ADF_SWEEP_TWO_LAUNCHES)
             return fail(ADF_EINVAL, "ADF_OPT_SWEEP_LAUNCH must be 0, 1 or 2");
@@ -731,13 +824,19 @@ static void launch_gf_gather(adf_solver* s, const Txs<T>& X, int t, hipStream_t 
                            gn[3], ng, s->gf_isrc, nout, nin, s->gf_h0, s->gf_h1, s->dom.z0, t, m, part);
 }
 
+static GfHist gf_hist(const adf_solver* s) {
+    return GfHist{s->gf_slot_first, s->gf_slot_next, s->gf_h0, s->gf_h1, s->gf_kind, s->dom.z0};
+}
+
+// tx_done: the velocity transceivers ran in k_tx_step.
 template <typename T>
-static void launch_after_v(adf_solver* s, const Solve<T>& S, const Txs<T>& X, int64_t it, hipStream_t st) {
+static void launch_after_v(adf_solver* s, const Solve<T>& S, const Txs<T>& X, int64_t it, hipStream_t st,
+                           bool tx_done = false) {
     const int it0 = (int)(it - 1);
     const int nt = (int)s->dom.nt;
     if (s->ntx) {
-        const GfHist H{s->gf_slot_first, s->gf_slot_next, s->gf_h0, s->gf_h1, s->gf_kind, s->dom.z0};
-        for (size_t L = 0; L + 1 < s->lev_v.size(); ++L) {
+        const GfHist H = gf_hist(s);
+        for (size_t L = 0; !tx_done && L + 1 < s->lev_v.size(); ++L) {
             const int n = s->lev_v[L + 1] - s->lev_v[L];
             if (n > 0)
                 hipLaunchKernelGGL((k_tx_v<T>), dim3((32 * n + 255) / 256), dim3(256), 0, st, S.Vx, S.Vy, S.Vz, S.nx, S.ny,
@@ -961,7 +1060,24 @@ static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int
         sweep_mark(s, st, mark);
         s->cur ^= 1;  // the new buffers become current
         Solve<T> S = make_solve<T>(s);
-        if (s->ntx) {
+        const bool one = s->ntx && s->txc_ok && s->tx_launch;
+        if (one) {
+            // every transceiver of the step in one launch (F still holds
+            // old = 0 and new = 1 buffers)
+            TxChain C{};
+            C.chain_p = s->txc_list;
+            C.chain_v = s->txc_list + s->txc_np;
+            C.rec = s->txc_list + s->txc_np + s->txc_nv;
+            C.fix = s->fix_cells;
+            C.n_fix = s->n_fix;
+            C.n_rec = s->txc_nrec;
+            C.nlp = (int)s->txc_lp.size() - 1;
+            C.nlv = (int)s->txc_lv.size() - 1;
+            for (int L = 0; L <= C.nlp; ++L) C.lp[L] = s->txc_lp[L];
+            for (int L = 0; L <= C.nlv; ++L) C.lv[L] = s->txc_lv[L];
+            hipLaunchKernelGGL((k_tx_step<T, EXACT>), dim3(1 + (C.n_rec + 7) / 8), dim3(256), 0, st, F, X, C, it0,
+                               gf_hist(s));
+        } else if (s->ntx) {
             launch_tx_p<T>(s, S, X, it0, st);
             if (s->n_fix) {
                 // recompute the velocity cells that read injected pressure
@@ -970,7 +1086,7 @@ static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int
                                    s->fix_cells, s->n_fix);
             }
         }
-        launch_after_v<T>(s, S, X, it, st);
+        launch_after_v<T>(s, S, X, it, st, one);
     } else {
         Solve<T> S = make_solve<T>(s);
         const dim3 blk(64, 4, 1);

@@ -225,6 +225,15 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
  *                        e.g. 256^3; slower where they fit); 0 (default):
  *                        ordinary stores.  Results are identical. */
 #define ADF_OPT_STORE_NT 9
+/*   ADF_OPT_TX_LAUNCH    1 (default): on the fused path, the transceivers
+ *                        of a step run in one launch when the ones that
+ *                        inject or read injected cells are few (at most 8
+ *                        levels and 512 entries per phase, 4096 fix-up
+ *                        cells): one workgroup runs those in order, the
+ *                        others record; 0: one launch per transceiver level
+ *                        and one for the velocity fix-up.  Results are
+ *                        identical. */
+#define ADF_OPT_TX_LAUNCH 10
 int adf_solver_set_option(adf_solver* s, int option, int64_t value);
 
 /* Mean device time (ms) of one field-update launch and the number of

@@ -21,13 +21,14 @@ TOL = {"f64": 1e-12, "ref": 1e-6, "fast": 1e-5}
 # Both kernel paths (two-pass P/V kernels and the fused z-marching sweep,
 # with several tile shapes, waves per workgroup, the pipelined and the
 # plane-at-a-time routines, and the three launch forms) must give the same
-# results, with ordinary and non-temporal stores.
-# (path, tile_y, block_z, waves, pipeline, launch, store_nt)
-PATHS = [("two-pass", 0, 0, 0, 0, 0, 0), ("fused", 0, 3, 1, 0, 0, 0), ("fused", 0, 0, 2, 1, 1, 0),
-         ("fused", 0, 5, 4, 1, 0, 1), ("fused", 0, 2, 1, 1, 2, 0)]
+# results, with ordinary and non-temporal stores, and with the transceivers
+# of a step in one launch or in one launch per level.
+# (path, tile_y, block_z, waves, pipeline, launch, store_nt, tx_launch)
+PATHS = [("two-pass", 0, 0, 0, 0, 0, 0, 1), ("fused", 0, 3, 1, 0, 0, 0, 0), ("fused", 0, 0, 2, 1, 1, 0, 1),
+         ("fused", 0, 5, 4, 1, 0, 1, 1), ("fused", 0, 2, 1, 1, 2, 0, 0)]
 
 
-@pytest.fixture(params=PATHS, ids=lambda p: f"{p[0]}-ty{p[1]}-kb{p[2]}-w{p[3]}-pf{p[4]}-l{p[5]}-nt{p[6]}")
+@pytest.fixture(params=PATHS, ids=lambda p: f"{p[0]}-ty{p[1]}-kb{p[2]}-w{p[3]}-pf{p[4]}-l{p[5]}-nt{p[6]}-tx{p[7]}")
 def path(request):
     return request.param
 
@@ -44,7 +45,8 @@ def _compare(d, txs, cpml, mask, kind, field=None, gf_pair=None, path=("auto", 0
     prec = "ref" if kind == "ref" else "fast"
     f, g, cp = gpu_run(d, txs, cpml, mask, dtype, prec, field=field, gf=None if gf_pair is None else gf_pair[1],
                        path=path[0], tile_y=path[1], block_z=path[2], waves=path[3], pipeline=path[4],
-                       launch=path[5], store_nt=path[6] if len(path) > 6 else 0)
+                       launch=path[5], store_nt=path[6] if len(path) > 6 else 0,
+                       tx_launch=path[7] if len(path) > 7 else 1)
     tol = TOL[kind]
     errs = {
         "p_rec": rel_l2(g.rec[0], ref.tx["rec"][0]),
@@ -218,7 +220,8 @@ def test_inject_inner_sources_after_setup(kind, path):
     ref.run()
     cp = A.make_cpml(d, cpml.n, cpml.rcoef, cpml.fc, T)
     s = A.Solver(d, A.dtype_code(T, "ref" if kind == "ref" else "fast"), cp)
-    s.set_path(path[0], path[1], path[2], path[3], pipeline=path[4], launch=path[5], store_nt=path[6])
+    s.set_path(path[0], path[1], path[2], path[3], pipeline=path[4], launch=path[5], store_nt=path[6],
+               tx_launch=path[7])
     s.set_transceivers(g)  # zero sources at packing time
     s.set_trace(A._lib.ADF_SRC_P, p_src)
     s.set_trace(A._lib.ADF_VN_SRC, vn)
