@@ -653,25 +653,29 @@ __global__ void k_fixup_v(Fused<T> F, const uint32_t* cells, int n) {
 // The transceiver work of one fused step in one launch (forward.jl:92-132;
 // k_tx_p levels, k_fixup_v and k_tx_v levels otherwise).  The chain is the
 // transceivers that inject, or whose stencils touch a cell written by an
-// injection or by the velocity fix-up, with the level offsets of their
-// order_p / order_v subsequences; all other transceivers only record.
+// injection or by the velocity fix-up.  It falls apart into groups that
+// share no written cell; each chain block takes some whole groups with
+// their fix-up cells, in their order_p / order_v level order.  All other
+// transceivers only record.
 constexpr int TXC_MAXLEV = 8;
+// per chain block: offsets into `list` of the pressure levels
+// [0, TXC_MAXLEV], of the velocity levels [TXC_MAXLEV + 1, 2 TXC_MAXLEV + 1],
+// then the first and end offsets of the fix-up cells
+constexpr int TXC_HDR = 2 * (TXC_MAXLEV + 1) + 2;
 struct TxChain {
-    const uint32_t* chain_p;  // order_p entries of the chain, level by level
-    const uint32_t* chain_v;  // order_v entries of the chain, level by level
-    const uint32_t* fix;      // velocity fix-up cells
-    const uint32_t* rec;      // indices of the record-only transceivers
-    int n_fix, n_rec;
-    int nlp, nlv;
-    int lp[TXC_MAXLEV + 1], lv[TXC_MAXLEV + 1];
+    const uint32_t* list;  // per chain block: order_p entries, order_v entries, fix-up cells
+    const int* hdr;        // TXC_HDR offsets per chain block
+    const uint32_t* rec;   // indices of the record-only transceivers
+    int nchain, n_rec;
+    int nlp, nlv;          // levels of the two orders
 };
 
-// Block 0 runs the chain in the reference order: the pressure levels, the
-// fix-up, the velocity levels, with a workgroup barrier after each (every
-// wave of the block is on one CU, so the barrier also orders the block's
-// global stores and loads).  Blocks 1.. take 8 record-only transceivers
-// each (32 lanes per transceiver): their records read cells nothing in
-// this launch writes, so they run beside the chain.
+// Blocks 0..nchain-1 run their groups in the reference order: pressure
+// levels, fix-up, velocity levels, with a workgroup barrier between phases
+// that hold work (every wave of a block is on one CU, so the barrier also
+// orders the block's global stores and loads).  The other blocks take 8
+// record-only transceivers each (32 lanes per transceiver): their records
+// read cells nothing in this launch writes, so they run beside the chain.
 template <typename T, bool EXACT>
 __global__ __launch_bounds__(256) void k_tx_step(Fused<T> F, Txs<T> X, TxChain C, int it0, GfHist H) {
     const int tid = (int)threadIdx.x, lw = tid & 63;
@@ -679,31 +683,41 @@ __global__ __launch_bounds__(256) void k_tx_step(Fused<T> F, Txs<T> X, TxChain C
     T* Vx = F.V1(0);
     T* Vy = F.V1(1);
     T* Vz = F.V1(2);
-    if (blockIdx.x == 0) {
+    if ((int)blockIdx.x < C.nchain) {
+        const int* h = C.hdr + (size_t)blockIdx.x * TXC_HDR;
+        bool pend = false;  // a phase with work ran since the last barrier
         for (int L = 0; L < C.nlp; ++L) {
-            const int end = C.lp[L + 1];
-            for (int b = C.lp[L]; b < end; b += 32) {
+            const int lo = h[L], hi = h[L + 1];
+            if (lo == hi) continue;
+            if (pend) __syncthreads();
+            for (int b = lo; b < hi; b += 32) {
                 const int g = b + (tid >> 3);
-                const bool live = g < end;
-                tx_p_entry<T>(P, F.nx, F.ny, F.nz, X, live ? C.chain_p[g] : 0u, live, lw, it0);
+                const bool live = g < hi;
+                tx_p_entry<T>(P, F.nx, F.ny, F.nz, X, live ? C.list[g] : 0u, live, lw, it0);
             }
-            __syncthreads();
+            pend = true;
         }
-        for (int b = 0; b < C.n_fix; b += 256)
-            if (b + tid < C.n_fix) fixup_cell<T, EXACT>(F, C.fix[b + tid]);
-        __syncthreads();
+        const int flo = h[2 * (TXC_MAXLEV + 1)], fhi = h[2 * (TXC_MAXLEV + 1) + 1];
+        if (flo < fhi) {
+            if (pend) __syncthreads();
+            for (int b = flo; b < fhi; b += 256)
+                if (b + tid < fhi) fixup_cell<T, EXACT>(F, C.list[b + tid]);
+            pend = true;
+        }
         for (int L = 0; L < C.nlv; ++L) {
-            const int end = C.lv[L + 1];
-            for (int b = C.lv[L]; b < end; b += 8) {
+            const int lo = h[TXC_MAXLEV + 1 + L], hi = h[TXC_MAXLEV + 2 + L];
+            if (lo == hi) continue;
+            if (pend) __syncthreads();
+            for (int b = lo; b < hi; b += 8) {
                 const int g = b + (tid >> 5);
-                const bool any = g < end;
-                tx_v_entry<T>(Vx, Vy, Vz, F.nx, F.ny, F.nz, X, any ? C.chain_v[g] : 0u, any, lw, it0, H);
+                const bool any = g < hi;
+                tx_v_entry<T>(Vx, Vy, Vz, F.nx, F.ny, F.nz, X, any ? C.list[g] : 0u, any, lw, it0, H);
             }
-            __syncthreads();
+            pend = true;
         }
         return;
     }
-    const int g = (int)(blockIdx.x - 1) * 8 + (tid >> 5);
+    const int g = ((int)blockIdx.x - C.nchain) * 8 + (tid >> 5);
     const bool any = g < C.n_rec;
     tx_record_entry<T>(P, Vx, Vy, Vz, F.nx, F.ny, F.nz, X, any ? (int)C.rec[g] : 0, any, lw, it0, H);
 }

@@ -79,11 +79,12 @@ struct adf_solver {
     // velocity cells next to pressure-injection stencils (fused path)
     uint32_t* fix_cells = nullptr;
     int n_fix = 0;
-    // one-launch transceiver step (k_tx_step): device lists chain_p |
-    // chain_v | record-only indices, level offsets of the two chains
+    // one-launch transceiver step (k_tx_step): the chain blocks' lists
+    // followed by the record-only indices (from txc_rec_off), and the chain
+    // blocks' offsets (TXC_HDR ints each)
     uint32_t* txc_list = nullptr;
-    int txc_np = 0, txc_nv = 0, txc_nrec = 0;
-    std::vector<int> txc_lp, txc_lv;
+    int* txc_hdr = nullptr;
+    int txc_nchain = 0, txc_nrec = 0, txc_rec_off = 0;
     bool txc_ok = false;  // the chain fits k_tx_step's limits
 
     // fused-sweep tile lists for tiling (tl_ty, tl_kb, tl_kbg, tl_w):

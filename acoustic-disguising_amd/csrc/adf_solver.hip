@@ -112,8 +112,9 @@ static void dfree(void* p) {
 static void free_txs(adf_solver* s) {
     dfree(s->tx_idx); dfree(s->tx_w); dfree(s->tx_src); dfree(s->tx_rec); dfree(s->tx_dir);
     dfree(s->tx_vn_src); dfree(s->tx_vn_rec); dfree(s->order_p); dfree(s->order_v); dfree(s->fix_cells);
-    dfree(s->gf_slot_first); dfree(s->gf_slot_next); dfree(s->txc_list);
+    dfree(s->gf_slot_first); dfree(s->gf_slot_next); dfree(s->txc_list); dfree(s->txc_hdr);
     s->gf_slot_first = s->gf_slot_next = nullptr;
+    s->txc_hdr = nullptr;
     s->fix_cells = nullptr;
     s->n_fix = 0;
     s->txc_list = nullptr;
@@ -254,22 +255,30 @@ static void injectors_from(adf_solver* s, int which, const void* data) {
     }
 }
 
-// The lists of the one-launch transceiver step (k_tx_step).  The chain is
-// every transceiver that injects (ip/iv), is a GF map output, or has a
-// stencil corner on a cell the step writes after the sweep: pressure cells
-// of the injectors, velocity cells of the fix-up (with their mirror cells)
-// and of the velocity injectors.  It keeps its order_p / order_v levels.
-// The others only record, from cells nothing in the launch writes, so their
-// records do not depend on the order.  Chains with more than TXC_MAXLEV
-// levels or more than TXC_MAX entries keep the per-level launches.
-static constexpr int TXC_MAX = 512, TXC_MAX_FIX = 4096;
+// The lists of the one-launch transceiver step (k_tx_step).  A cell is
+// written after the sweep when an injector's stencil covers it (pressure
+// cells of ip, velocity cells of iv) or when it is a fix-up cell (with its
+// mirror cell).  The chain is every transceiver that injects, is a GF map
+// output, or has a stencil corner on a written cell; the others only
+// record, from cells nothing in the launch writes, so their records do not
+// depend on the order.  Transceivers sharing a written cell (and the
+// injectors a fix-up cell reads) form groups; the transceiver order
+// (levels) and the fix-up only constrain members of one group, so groups
+// run independently, whole groups packed into chain blocks of up to
+// TXC_BLK_TX transceivers and TXC_BLK_FIX fix-up cells.  A group with more
+// than TXC_MAX transceivers, or more than TXC_MAXLEV levels, keeps the
+// per-level launches.
+static constexpr int TXC_MAX = 64, TXC_MAX_FIX = 4096, TXC_BLK_TX = 8, TXC_BLK_FIX = 256;
 static int build_tx_chain(adf_solver* s, const std::vector<uint8_t>& ip, const std::vector<uint8_t>& iv,
                           const std::vector<uint32_t>& order_p, const std::vector<uint32_t>& order_v,
                           const std::vector<uint32_t>& fix) {
     dfree(s->txc_list);
+    dfree(s->txc_hdr);
     s->txc_list = nullptr;
+    s->txc_hdr = nullptr;
     s->txc_ok = false;
     const int64_t ntx = s->ntx;
+    if ((int)s->lev_p.size() - 1 > TXC_MAXLEV || (int)s->lev_v.size() - 1 > TXC_MAXLEV) return ADF_OK;
     const int nx = (int)s->dom.nx, ny = (int)s->dom.ny, nz = (int)s->dom.nz;
     const int64_t n3 = (int64_t)nx * ny * nz;
     auto corners = [&](int64_t is, int ch, int64_t* out) {
@@ -280,62 +289,129 @@ static int build_tx_chain(adf_solver* s, const std::vector<uint8_t>& ip, const s
             out[q] = (int64_t)(I - 1 + (q & 1)) + (int64_t)nx * ((J - 1 + ((q >> 1) & 1)) + (int64_t)ny * (K - 1 + (q >> 2)));
         return true;
     };
-    // written cells, keyed channel * n3 + cell (channel 0 = P, 1..3 = Vx..Vz)
-    std::unordered_set<int64_t> written;
+    std::vector<int> par((size_t)ntx);
+    for (int64_t i = 0; i < ntx; ++i) par[i] = (int)i;
+    auto find = [&](int x) {
+        while (par[x] != x) x = par[x] = par[par[x]];
+        return x;
+    };
+    auto unite = [&](int a, int b) {
+        a = find(a);
+        b = find(b);
+        if (a != b) par[std::max(a, b)] = std::min(a, b);
+    };
+    // written cells, keyed channel * n3 + cell (channel 0 = P, 1..3 = Vx..Vz),
+    // each with one writer; writers of one cell are united
+    std::unordered_map<int64_t, int> owner;
+    auto own = [&](int64_t key, int is) {
+        auto it = owner.find(key);
+        if (it == owner.end()) owner.emplace(key, is);
+        else unite(it->second, is);
+    };
     int64_t cs[8];
     for (int64_t is = 0; is < ntx; ++is) {
         if (ip[is] && corners(is, 0, cs))
-            for (int q = 0; q < 8; ++q) written.insert(cs[q]);
+            for (int q = 0; q < 8; ++q) own(cs[q], (int)is);
         if (iv[is])
             for (int ch = 1; ch <= 3; ++ch)
                 if (corners(is, ch, cs))
-                    for (int q = 0; q < 8; ++q) written.insert(ch * n3 + cs[q]);
+                    for (int q = 0; q < 8; ++q) own(ch * n3 + cs[q], (int)is);
     }
-    for (uint32_t e : fix) {
-        const int64_t ax = e >> 30, c = e & 0x3fffffffu;
-        written.insert((ax + 1) * n3 + c);
-        if (ax == 1 && c >= nx) written.insert(2 * n3 + c - nx);                        // Vy mirror
-        if (ax == 2 && c >= (int64_t)nx * ny) written.insert(3 * n3 + c - (int64_t)nx * ny);  // Vz mirror
+    // a fix-up cell belongs with the injectors of the pressure cells it
+    // reads (at least one of its two is injected: the list is built so)
+    std::vector<int> fix_of(fix.size(), -1);
+    for (size_t x = 0; x < fix.size(); ++x) {
+        const int ax = (int)(fix[x] >> 30);
+        const int64_t c = fix[x] & 0x3fffffffu;
+        const int64_t st = ax == 0 ? 1 : (ax == 1 ? nx : (int64_t)nx * ny);
+        for (int64_t pc : {c, c - st}) {
+            auto it = owner.find(pc);
+            if (it == owner.end()) continue;
+            if (fix_of[x] < 0) fix_of[x] = it->second;
+            else unite(fix_of[x], it->second);
+        }
+        if (fix_of[x] < 0) return ADF_OK;  // not expected; keep the level launches
+        own((ax + 1) * n3 + c, fix_of[x]);
+        const int64_t j = (c / nx) % ny, k = c / ((int64_t)nx * ny);
+        if (ax == 1 && j == 1) own(2 * n3 + c - nx, fix_of[x]);                 // Vy mirror
+        if (ax == 2 && k == 1) own(3 * n3 + c - (int64_t)nx * ny, fix_of[x]);   // Vz mirror
     }
     std::vector<uint8_t> dep((size_t)ntx, 0);
     for (uint32_t e : order_v)
         if (e & TX_GF_OUT) dep[e & TX_INDEX] = 1;
     for (int64_t is = 0; is < ntx; ++is) {
         if (ip[is] || iv[is]) dep[is] = 1;
-        for (int ch = 0; ch <= 3 && !dep[is]; ++ch)
+        for (int ch = 0; ch <= 3; ++ch)
             if (corners(is, ch, cs))
-                for (int q = 0; q < 8; ++q)
-                    if (written.count(ch * n3 + cs[q])) { dep[is] = 1; break; }
+                for (int q = 0; q < 8; ++q) {
+                    auto it = owner.find(ch * n3 + cs[q]);
+                    if (it != owner.end()) {
+                        dep[is] = 1;
+                        unite((int)is, it->second);
+                    }
+                }
     }
-    // the chain's subsequences of the two level orders, empty levels dropped
-    auto chain = [&](const std::vector<uint32_t>& order, const std::vector<int>& lev, std::vector<uint32_t>& out,
-                     std::vector<int>& off) {
-        off.assign(1, 0);
-        for (size_t L = 0; L + 1 < lev.size(); ++L) {
-            for (int x = lev[L]; x < lev[L + 1]; ++x)
-                if (dep[order[x] & TX_INDEX]) out.push_back(order[x]);
-            if ((int)out.size() > off.back()) off.push_back((int)out.size());
-        }
-    };
-    std::vector<uint32_t> cp, cv, rec;
-    chain(order_p, s->lev_p, cp, s->txc_lp);
-    chain(order_v, s->lev_v, cv, s->txc_lv);
+    // group sizes, then whole groups packed into chain blocks in the order
+    // of their first member
+    std::vector<int> gsize((size_t)ntx, 0), gfix((size_t)ntx, 0), blk((size_t)ntx, -1);
     for (int64_t is = 0; is < ntx; ++is)
-        if (!dep[is]) rec.push_back((uint32_t)is);
-    if ((int)s->txc_lp.size() - 1 > TXC_MAXLEV || (int)s->txc_lv.size() - 1 > TXC_MAXLEV || (int)cp.size() > TXC_MAX ||
-        (int)cv.size() > TXC_MAX || (int)fix.size() > TXC_MAX_FIX)
-        return ADF_OK;
-    s->txc_np = (int)cp.size();
-    s->txc_nv = (int)cv.size();
-    s->txc_nrec = (int)rec.size();
-    std::vector<uint32_t> all;
-    all.insert(all.end(), cp.begin(), cp.end());
-    all.insert(all.end(), cv.begin(), cv.end());
-    all.insert(all.end(), rec.begin(), rec.end());
-    if (!all.empty()) {
-        int rc = dmalloc((void**)&s->txc_list, all.size() * 4, "transceiver chain");
+        if (dep[is]) gsize[find((int)is)]++;
+    for (size_t x = 0; x < fix.size(); ++x) gfix[find(fix_of[x])]++;
+    int nb = 0, btx = 0, bfix = 0;
+    for (int64_t is = 0; is < ntx; ++is) {
+        if (!dep[is]) continue;
+        const int r = find((int)is);
+        if (blk[r] >= 0) continue;
+        if (gsize[r] > TXC_MAX || gfix[r] > TXC_MAX_FIX) return ADF_OK;
+        if (nb == 0 || btx + gsize[r] > TXC_BLK_TX || bfix + gfix[r] > TXC_BLK_FIX) {
+            ++nb;
+            btx = bfix = 0;
+        }
+        blk[r] = nb - 1;
+        btx += gsize[r];
+        bfix += gfix[r];
+    }
+    // per block: pressure entries by level, velocity entries by level, fix-up cells
+    std::vector<std::vector<uint32_t>> bp((size_t)nb * TXC_MAXLEV), bv((size_t)nb * TXC_MAXLEV), bf((size_t)nb);
+    const int nlp = (int)s->lev_p.size() - 1, nlv = (int)s->lev_v.size() - 1;
+    for (int L = 0; L < nlp; ++L)
+        for (int x = s->lev_p[L]; x < s->lev_p[L + 1]; ++x)
+            if (dep[order_p[x] & TX_INDEX]) bp[(size_t)blk[find((int)(order_p[x] & TX_INDEX))] * TXC_MAXLEV + L].push_back(order_p[x]);
+    for (int L = 0; L < nlv; ++L)
+        for (int x = s->lev_v[L]; x < s->lev_v[L + 1]; ++x)
+            if (dep[order_v[x] & TX_INDEX]) bv[(size_t)blk[find((int)(order_v[x] & TX_INDEX))] * TXC_MAXLEV + L].push_back(order_v[x]);
+    for (size_t x = 0; x < fix.size(); ++x) bf[blk[find(fix_of[x])]].push_back(fix[x]);
+    std::vector<uint32_t> list;
+    std::vector<int> hdr((size_t)nb * TXC_HDR, 0);
+    for (int b = 0; b < nb; ++b) {
+        int* h = &hdr[(size_t)b * TXC_HDR];
+        for (int L = 0; L <= TXC_MAXLEV; ++L) {
+            h[L] = (int)list.size();
+            if (L < TXC_MAXLEV) list.insert(list.end(), bp[(size_t)b * TXC_MAXLEV + L].begin(), bp[(size_t)b * TXC_MAXLEV + L].end());
+        }
+        for (int L = 0; L <= TXC_MAXLEV; ++L) {
+            h[TXC_MAXLEV + 1 + L] = (int)list.size();
+            if (L < TXC_MAXLEV) list.insert(list.end(), bv[(size_t)b * TXC_MAXLEV + L].begin(), bv[(size_t)b * TXC_MAXLEV + L].end());
+        }
+        h[2 * (TXC_MAXLEV + 1)] = (int)list.size();
+        list.insert(list.end(), bf[b].begin(), bf[b].end());
+        h[2 * (TXC_MAXLEV + 1) + 1] = (int)list.size();
+    }
+    const int nlist = (int)list.size();
+    for (int64_t is = 0; is < ntx; ++is)
+        if (!dep[is]) list.push_back((uint32_t)is);
+    s->txc_nchain = nb;
+    s->txc_nrec = (int)list.size() - nlist;
+    s->txc_rec_off = nlist;
+    if (!list.empty()) {
+        int rc = dmalloc((void**)&s->txc_list, list.size() * 4, "transceiver chain");
         if (rc) return rc;
-        HIPCHK(hipMemcpy(s->txc_list, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(s->txc_list, list.data(), list.size() * 4, hipMemcpyHostToDevice));
+    }
+    if (nb) {
+        int rc = dmalloc((void**)&s->txc_hdr, hdr.size() * 4, "transceiver chain");
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(s->txc_hdr, hdr.data(), hdr.size() * 4, hipMemcpyHostToDevice));
     }
     s->txc_ok = true;
     return ADF_OK;
@@ -1065,18 +1141,16 @@ static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int
             // every transceiver of the step in one launch (F still holds
             // old = 0 and new = 1 buffers)
             TxChain C{};
-            C.chain_p = s->txc_list;
-            C.chain_v = s->txc_list + s->txc_np;
-            C.rec = s->txc_list + s->txc_np + s->txc_nv;
-            C.fix = s->fix_cells;
-            C.n_fix = s->n_fix;
+            C.list = s->txc_list;
+            C.hdr = s->txc_hdr;
+            C.rec = s->txc_list + s->txc_rec_off;
+            C.nchain = s->txc_nchain;
             C.n_rec = s->txc_nrec;
-            C.nlp = (int)s->txc_lp.size() - 1;
-            C.nlv = (int)s->txc_lv.size() - 1;
-            for (int L = 0; L <= C.nlp; ++L) C.lp[L] = s->txc_lp[L];
-            for (int L = 0; L <= C.nlv; ++L) C.lv[L] = s->txc_lv[L];
-            hipLaunchKernelGGL((k_tx_step<T, EXACT>), dim3(1 + (C.n_rec + 7) / 8), dim3(256), 0, st, F, X, C, it0,
-                               gf_hist(s));
+            C.nlp = (int)s->lev_p.size() - 1;
+            C.nlv = (int)s->lev_v.size() - 1;
+            const int nblk = C.nchain + (C.n_rec + 7) / 8;
+            if (nblk)
+                hipLaunchKernelGGL((k_tx_step<T, EXACT>), dim3(nblk), dim3(256), 0, st, F, X, C, it0, gf_hist(s));
         } else if (s->ntx) {
             launch_tx_p<T>(s, S, X, it0, st);
             if (s->n_fix) {
