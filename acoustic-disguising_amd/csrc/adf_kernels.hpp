@@ -316,12 +316,9 @@ struct GfHist {
     double z0;
 };
 
-// Velocity transceivers of one level (forward.jl:111-115), then v2vn!
-// (forward.jl:120) and vn2v! into it+1 (forward.jl:130-132) for the same
-// transceiver, and the GF history of the step.  Entries with bit 30 set
-// are outputs of a GF map with write-back: their vn2v! runs after the
-// extrapolation writes their sources (k_gf_near).  32 lanes per
-// transceiver: channel x, y, z on lanes 0-7, 8-15, 16-23.
+// order_p / order_v entries: transceiver index | TX_INJ (the transceiver
+// injects on that phase) | TX_GF_OUT (order_v: an output of a GF map with
+// write-back, whose vn2v! runs after the extrapolation, in k_gf_near).
 constexpr uint32_t TX_INJ = 1u << 31, TX_GF_OUT = 1u << 30, TX_INDEX = TX_GF_OUT - 1;
 
 // The GF history of step it0 for transceiver is from its records p and vn.

@@ -229,3 +229,32 @@ def test_inject_inner_sources_after_setup(kind, path):
     for which, r in ((0, ref.tx["rec"][0]), (4, ref.tx["vn_rec"]), (6, ref.tx["src"][1]), (8, ref.tx["src"][3])):
         e = rel_l2(s.get_trace(which), r)
         assert e <= TOL[kind], (which, e)
+
+
+def test_injection_surface_groups_one_launch_equals_levels():
+    # A KH-style surface: a pressure (q) and a velocity (f) injector at each
+    # of 60 points (~4.6 cells apart) and 30 recorders further out.  The
+    # injectors fall into small groups that share written cells, packed
+    # several per chain workgroup of the one-launch transceiver step; the
+    # result must be the per-level launches' bit for bit and the oracle's.
+    from cases import domain_cells
+    d = domain_cells(36, 0.00686, 50)
+    rng = np.random.default_rng(11)
+    pts, nrm = A.fibonacci_sphere(60, 10 * d.dx)
+    rp, rn = A.fibonacci_sphere(30, 14 * d.dx)
+    txs = [A.make_transceiver(d, pts[i], [1.0, 0.0, 0.0, 0.0], rng.standard_normal(d.nt) * 1e-3) for i in range(60)]
+    txs += [A.make_transceiver(d, pts[i], np.r_[0.0, nrm[i]], rng.standard_normal(d.nt) * 1e-6) for i in range(60)]
+    txs += [A.make_transceiver(d, rp[i], np.r_[0.0, rn[i]], np.zeros(d.nt)) for i in range(30)]
+    cpml = A.make_cpml(d, 5, 1e-4, 2000.0)
+    mask = A.build_update_mask(d, "sphere", [4 * d.dx])
+    ref = oracle_solver(O, d, txs, cpml, mask, "f64")
+    ref.run()
+    runs = [gpu_run(d, txs, cpml, mask, np.float64, "fast", path="fused", tx_launch=t) for t in (1, 0)]
+    (f1, g1, _), (f0, g0, _) = runs
+    assert np.array_equal(f1, f0)
+    assert np.array_equal(g1.rec, g0.rec) and np.array_equal(g1.vn_rec, g0.vn_rec)
+    assert np.array_equal(g1.src, g0.src)
+    errs = [rel_l2(g1.rec[c], ref.tx["rec"][c]) for c in range(4)]
+    errs += [rel_l2(g1.vn_rec, ref.tx["vn_rec"]), rel_l2(g1.src, ref.tx["src"]), rel_l2(f1, ref.field)]
+    print("f64 vs oracle", [float(f"{e:.3g}") for e in errs], flush=True)
+    assert max(errs) <= TOL["f64"], errs
