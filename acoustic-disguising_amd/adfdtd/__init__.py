@@ -7,6 +7,7 @@ package is the host-side mirror of the reference's Julia API.
 from .simulation import (CHANNELS, Cpml, Domain, Transceiver, TransceiverGrid, compute_cpml_coefficients,
                          coords, initial_field, make_cpml, make_domain, make_transceiver, p_rec, p_src,
                          reset_cpml_memory, run_fdtd, t, transceivers_to_grid, trilinear_interpolation, vn_rec,
+                         concat_grids,
                          vn_src, x, xv, y, yv, z, zv)
 from .geometry import build_scatterer, build_update_mask, outside_predicate
 from .illumination import c_for_n, fibonacci_sphere, illumination_points, impulsive_wavelet, r2_sphere, ricker_wavelet

@@ -20,7 +20,7 @@ import numpy as np
 from . import _lib as L
 from .geometry import SCATTERER_DEFAULTS
 from .illumination import illumination_points, impulsive_wavelet
-from .simulation import Domain, make_cpml, make_transceiver, transceivers_to_grid
+from .simulation import Domain, concat_grids, make_cpml, make_transceiver, transceivers_to_grid
 from .solver import Solver, dtype_code
 
 _SHAPE_PARAM = {"sphere": "sphere_radius", "cube": "cube_length", "cross": "cross_length"}
@@ -81,13 +81,16 @@ def impulsive_gfs(dom: Domain, cfg: dict, scatterer: str = "cross", precision: s
     todo = [i for i in range(n_src) if not done(i)]
     code = dtype_code(dtype, precision)
 
+    # every run packs the same receivers behind its source: pack them once
+    rec_grid = transceivers_to_grid(dom, receivers, dtype)
+
     def make_solver(i, kind):
         direction = [1.0, 0.0, 0.0, 0.0] if kind == "p" else np.r_[0.0, src_nrm[i]]
         src = make_transceiver(dom, src_pos[i], direction, srctf)
         s = Solver(dom, code, make_cpml(dom, npml, rcoef, fc, dtype))
         if params is not None:
             s.build_mask(scatterer, params)
-        s.set_transceivers(transceivers_to_grid(dom, [src] + receivers, dtype))
+        s.set_transceivers(concat_grids(transceivers_to_grid(dom, [src], dtype), rec_grid))
         return s
 
     for c0 in range(0, len(todo), max(1, int(sources_per_batch))):

@@ -217,6 +217,17 @@ class TransceiverGrid:
         return self.direction.shape[0]
 
 
+def concat_grids(a: TransceiverGrid, b: TransceiverGrid) -> TransceiverGrid:
+    """The packing of a's transceivers followed by b's: transceivers_to_grid
+    packs each transceiver on its own (simulation.jl:387-431), so this equals
+    packing the concatenated list."""
+    return TransceiverGrid(idx=np.concatenate([a.idx, b.idx], axis=1), w=np.concatenate([a.w, b.w], axis=1),
+                           src=np.concatenate([a.src, b.src], axis=1), rec=np.concatenate([a.rec, b.rec], axis=1),
+                           direction=np.concatenate([a.direction, b.direction], axis=0),
+                           vn_src=np.concatenate([a.vn_src, b.vn_src], axis=0),
+                           vn_rec=np.concatenate([a.vn_rec, b.vn_rec], axis=0))
+
+
 def p_src(t: TransceiverGrid) -> np.ndarray:
     """src/simulation.jl:449-452 accessors; rows are transceivers."""
     return t.src[0]

@@ -59,6 +59,7 @@ extern "C" int adf_ctx_destroy(adf_ctx* c) {
     hipStreamSynchronize(c->side);
     hipStreamDestroy(c->side);
     hipStreamDestroy(c->stream);
+    device_pool_release(c->device);
     delete c;
     return ADF_OK;
 }

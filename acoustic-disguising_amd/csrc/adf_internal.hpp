@@ -183,6 +183,8 @@ struct adf_batch {
 };
 
 namespace adf {
+// Frees the device allocations pooled for reuse on device dev.
+void device_pool_release(int dev);
 // A change to a member's schedule, mask or tiling invalidates its batch's
 // device tables.
 inline void touch(adf_solver* s) {

@@ -261,6 +261,7 @@ extern "C" int adf_server_create(int device, uint64_t hbm_budget_bytes, adf_serv
     S->budget = hbm_budget_bytes;
     if (!S->budget) {
         size_t fr = 0, tot = 0;
+        device_pool_release(device);  // pooled blocks count as free memory here
         if (hipMemGetInfo(&fr, &tot) == hipSuccess) S->budget = (uint64_t)(0.85 * (double)fr);
     }
     S->th = std::thread(loop, S);

@@ -6,6 +6,7 @@
 #include <array>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <type_traits>
 #include <unordered_map>
@@ -26,16 +27,84 @@ static void to_type(const double* in, T* out, size_t n) {
     for (size_t i = 0; i < n; ++i) out[i] = (T)in[i];
 }
 
+// Device blocks that solvers, batches and servers free are kept per device
+// and size for the next allocation of the same size.  hipFree waits for the
+// device and costs ~0.1 ms per block, which made creating and destroying
+// many solves of one shape (one per source in impulsive_gfs, one per served
+// request) a visible part of their time.  hipFree also waits for the device
+// to go idle; the pool keeps that guarantee at reuse instead: each pooled
+// block carries the count of frees at its free, and a block freed after the
+// last device synchronization is handed out only after another one.  The
+// pool holds at most POOL_MAX bytes; an allocation that fails releases the
+// device's pool and tries again.  Reused blocks are not zeroed: every
+// buffer is written or cleared after allocation.
+namespace {
+std::mutex pool_mu;
+std::unordered_map<void*, std::pair<int, size_t>> pool_live;  // block -> (device, bytes)
+std::map<std::pair<int, size_t>, std::vector<std::pair<void*, uint64_t>>> pool_free;  // (block, free count)
+size_t pool_bytes = 0;
+uint64_t pool_frees = 0;   // blocks pooled so far
+uint64_t pool_synced = 0;  // pool_frees at the start of the last device synchronization
+constexpr size_t POOL_MAX = (size_t)32 << 30;
+
+void pool_release_locked(int dev) {
+    for (auto it = pool_free.begin(); it != pool_free.end();) {
+        if (it->first.first != dev) {
+            ++it;
+            continue;
+        }
+        for (auto& b : it->second) {
+            (void)hipFree(b.first);
+            pool_bytes -= it->first.second;
+        }
+        it = pool_free.erase(it);
+    }
+}
+}  // namespace
+
+// Frees the pooled blocks of device dev (the current device).
+void device_pool_release(int dev) {
+    std::lock_guard<std::mutex> lk(pool_mu);
+    pool_release_locked(dev);
+}
+
 static int dmalloc(void** p, size_t bytes, const char* what) {
-    if (bytes == 0) {
-        *p = nullptr;
-        return ADF_OK;
+    *p = nullptr;
+    if (bytes == 0) return ADF_OK;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    {
+        std::unique_lock<std::mutex> lk(pool_mu);
+        auto it = pool_free.find({dev, bytes});
+        if (it != pool_free.end() && !it->second.empty()) {
+            const std::pair<void*, uint64_t> b = it->second.back();
+            it->second.pop_back();
+            pool_bytes -= bytes;
+            pool_live[b.first] = {dev, bytes};
+            if (b.second > pool_synced) {  // freed since the last synchronization
+                const uint64_t upto = pool_frees;
+                lk.unlock();
+                (void)hipDeviceSynchronize();
+                lk.lock();
+                pool_synced = std::max(pool_synced, upto);
+            }
+            *p = b.first;
+            return ADF_OK;
+        }
     }
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
+        (void)hipGetLastError();
+        device_pool_release(dev);
+        e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess) {
         *p = nullptr;
+        (void)hipGetLastError();
         return fail(ADF_ENOMEM, std::string("hipMalloc ") + what + ": " + hipGetErrorString(e));
     }
+    std::lock_guard<std::mutex> lk(pool_mu);
+    pool_live[*p] = {dev, bytes};
     return ADF_OK;
 }
 
@@ -106,7 +175,21 @@ int solver_alloc(adf_solver* s, const adf_cpml* cp) {
 }
 
 static void dfree(void* p) {
-    if (p) hipFree(p);
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(pool_mu);
+        auto it = pool_live.find(p);
+        if (it != pool_live.end()) {
+            const std::pair<int, size_t> key = it->second;
+            pool_live.erase(it);
+            if (pool_bytes + key.second <= POOL_MAX) {
+                pool_free[key].push_back({p, ++pool_frees});
+                pool_bytes += key.second;
+                return;
+            }
+        }
+    }
+    (void)hipFree(p);
 }
 
 static void free_txs(adf_solver* s) {
@@ -1288,7 +1371,8 @@ int solver_slice(adf_solver* s, int axis, int64_t index, void* out) {
                            (float*)buf, nx, ny, nz, axis, (int)(index - 1));
     hipError_t e = hipMemcpyAsync(out, buf, n * s->es, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(buf);
+    (void)hipStreamSynchronize(st);
+    dfree(buf);
     if (e != hipSuccess) return fail(ADF_EDEVICE, std::string("slice copy: ") + hipGetErrorString(e));
     return ADF_OK;
 }
@@ -1301,7 +1385,7 @@ int solver_build_mask_device(adf_solver* s, int shape, const double* params, con
     uint32_t* solid = nullptr;
     int rc;
     if ((rc = dmalloc((void**)&dbuf, (size_t)(4 + nx + ny + nz) * 8, "mask coords"))) return rc;
-    if ((rc = dmalloc((void**)&solid, pl * 4, "mask solid"))) { hipFree(dbuf); return rc; }
+    if ((rc = dmalloc((void**)&solid, pl * 4, "mask solid"))) { dfree(dbuf); return rc; }
     std::vector<double> h(4 + nx + ny + nz, 0.0);
     const int np = shape == 4 ? 3 : 1;
     for (int i = 0; i < np; ++i) h[i] = params[i];
@@ -1317,8 +1401,9 @@ int solver_build_mask_device(adf_solver* s, int shape, const double* params, con
                            ny, nz, nwx);
         e = hipStreamSynchronize(st);
     }
-    hipFree(dbuf);
-    hipFree(solid);
+    (void)hipStreamSynchronize(st);
+    dfree(dbuf);
+    dfree(solid);
     if (e != hipSuccess) return fail(ADF_EDEVICE, std::string("mask build: ") + hipGetErrorString(e));
     return ADF_OK;
 }

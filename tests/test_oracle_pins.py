@@ -484,3 +484,20 @@ def test_gf_denoise_uses_the_signed_maximum():
     assert d.tolist() == [[1.0, 0.0, 0.0, 1e-4, -2.0, 0.0]]
     neg = np.array([[-1.0, -1e-9, -3.0]])
     assert np.array_equal(O.gf_denoise(neg), neg)
+
+
+def test_concat_grids_equals_packing_the_list():
+    # impulsive_gfs packs the shared receivers once and puts each run's source
+    # in front: the same arrays as packing [source] + receivers
+    import adfdtd as A
+    d = A.make_domain(20 * 5.7735e-6, 0.21609, 0.21609, 0.21609, 64, 64, 64, 343.0, 1.21, 0.5)
+    pts, nrm = A.fibonacci_sphere(30, 20 * d.dx)
+    rng = np.random.default_rng(5)
+    rec = [A.make_transceiver(d, pts[i], np.r_[0.0, nrm[i]], np.zeros(d.nt)) for i in range(30)]
+    src = A.make_transceiver(d, np.array([1.3, -0.4, 2.2]) * d.dx, [1.0, 0.0, 0.0, 0.0], rng.standard_normal(d.nt))
+    for T in (np.float32, np.float64):
+        a = A.transceivers_to_grid(d, [src] + rec, T)
+        b = A.concat_grids(A.transceivers_to_grid(d, [src], T), A.transceivers_to_grid(d, rec, T))
+        for k in ("idx", "w", "src", "rec", "direction", "vn_src", "vn_rec"):
+            x, y = getattr(a, k), getattr(b, k)
+            assert x.dtype == y.dtype and np.array_equal(x, y), k
