@@ -684,10 +684,28 @@ __global__ __launch_bounds__(256) void k_tx_step(Fused<T> F, Txs<T> X, TxChain C
     T* Vy = F.V1(1);
     T* Vz = F.V1(2);
     if ((int)blockIdx.x < C.nchain) {
-        const int* h = C.hdr + (size_t)blockIdx.x * TXC_HDR;
+        // the block's offsets: one vector load, then lane reads (a scalar
+        // load per offset would wait a memory round trip each)
+        const int hv = lw < TXC_HDR ? C.hdr[(size_t)blockIdx.x * TXC_HDR + lw] : 0;
+        auto h = [&](int i) { return __builtin_amdgcn_readlane(hv, i); };
+        // Pressure entries fill the block from thread 0, velocity entries
+        // from thread 64 and fix-up cells from thread 128 (rotated), so with
+        // few entries the first velocity pass's loads (entry, index, weight,
+        // record, source) and the fix-up entries are issued at the start by
+        // waves the pressure phase leaves idle, beside its loads.  None of
+        // them reads a cell this launch writes.
+        const int vt = (tid + 192) & 255, ft = (tid + 128) & 255;
+        int lv0 = 0;
+        while (lv0 < C.nlv && h(TXC_MAXLEV + 1 + lv0) == h(TXC_MAXLEV + 2 + lv0)) ++lv0;
+        const int vlo = lv0 < C.nlv ? h(TXC_MAXLEV + 1 + lv0) : 0;
+        const int vhi = lv0 < C.nlv ? h(TXC_MAXLEV + 2 + lv0) : 0;
+        const int flo = h(2 * (TXC_MAXLEV + 1)), fhi = h(2 * (TXC_MAXLEV + 1) + 1);
+        const bool vany0 = vlo + (vt >> 5) < vhi;
+        const TxV<T> v0 = tx_v_load<T>(F.nx, F.ny, F.nz, X, vany0 ? C.list[vlo + (vt >> 5)] : 0u, vany0, lw, it0);
+        const uint32_t fe0 = flo + ft < fhi ? C.list[flo + ft] : 0u;
         bool pend = false;  // a phase with work ran since the last barrier
         for (int L = 0; L < C.nlp; ++L) {
-            const int lo = h[L], hi = h[L + 1];
+            const int lo = h(L), hi = h(L + 1);
             if (lo == hi) continue;
             if (pend) __syncthreads();
             for (int b = lo; b < hi; b += 32) {
@@ -697,21 +715,23 @@ __global__ __launch_bounds__(256) void k_tx_step(Fused<T> F, Txs<T> X, TxChain C
             }
             pend = true;
         }
-        const int flo = h[2 * (TXC_MAXLEV + 1)], fhi = h[2 * (TXC_MAXLEV + 1) + 1];
         if (flo < fhi) {
             if (pend) __syncthreads();
             for (int b = flo; b < fhi; b += 256)
-                if (b + tid < fhi) fixup_cell<T, EXACT>(F, C.list[b + tid]);
+                if (b + ft < fhi) fixup_cell<T, EXACT>(F, b == flo ? fe0 : C.list[b + ft]);
             pend = true;
         }
         for (int L = 0; L < C.nlv; ++L) {
-            const int lo = h[TXC_MAXLEV + 1 + L], hi = h[TXC_MAXLEV + 2 + L];
+            const int lo = h(TXC_MAXLEV + 1 + L), hi = h(TXC_MAXLEV + 2 + L);
             if (lo == hi) continue;
             if (pend) __syncthreads();
             for (int b = lo; b < hi; b += 8) {
-                const int g = b + (tid >> 5);
+                const int g = b + (vt >> 5);
                 const bool any = g < hi;
-                tx_v_entry<T>(Vx, Vy, Vz, F.nx, F.ny, F.nz, X, any ? C.list[g] : 0u, any, lw, it0, H);
+                if (L == lv0 && b == lo)
+                    tx_v_apply<T>(Vx, Vy, Vz, X, v0, any, lw, it0, H);
+                else
+                    tx_v_entry<T>(Vx, Vy, Vz, F.nx, F.ny, F.nz, X, any ? C.list[g] : 0u, any, lw, it0, H);
             }
             pend = true;
         }

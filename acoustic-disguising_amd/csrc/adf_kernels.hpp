@@ -258,25 +258,47 @@ struct Txs {
 // equals the reference's record-then-inject loop value for value.  `live`
 // lanes belong to a real entry; all lanes must call this (shuffles).
 // Returns the new record (the old one when the stencil is skipped).
+// tx_lane_load does the loads that do not read the field (index, weight,
+// old record, source sample); tx_lane_apply reads the corner and records /
+// injects, so a caller may issue the first part ahead.
+template <typename T>
+struct TxLane {
+    int c;    // the lane's corner cell
+    bool ok;  // the stencil lies inside the grid
+    T w, r0, sv;
+};
+
+template <typename T>
+__device__ __forceinline__ TxLane<T> tx_lane_load(const int32_t* id3, int nx, int ny, int nz, const T* w8,
+                                                  const T* rec, const T* src, bool inj, bool live, int q) {
+    TxLane<T> L;
+    int I = 0, J = 0, K = 0;
+    if (live) { I = id3[0]; J = id3[1]; K = id3[2]; }
+    L.ok = live && I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz;
+    L.c = (I - 1) + nx * ((J - 1) + ny * (K - 1)) + (q & 1) + nx * ((q >> 1) & 1) + nx * ny * (q >> 2);
+    L.w = L.ok ? w8[q] : (T)0;
+    L.r0 = live ? *rec : (T)0;
+    L.sv = (L.ok && inj) ? *src : (T)0;
+    return L;
+}
+
+template <typename T>
+__device__ __forceinline__ T tx_lane_apply(T* F, const TxLane<T>& L, T* rec, bool inj, int q, int g0) {
+    const T f = L.ok ? F[L.c] : (T)0;
+    const T prod = L.w * f;
+    T r = L.r0;
+#pragma unroll
+    for (int qq = 0; qq < 8; ++qq) r = r + __shfl(prod, g0 + qq, 64);
+    if (!L.ok) r = L.r0;
+    if (L.ok && q == 0) *rec = r;
+    if (L.ok && inj) F[L.c] = f + L.w * L.sv;
+    return r;
+}
+
 template <typename T>
 __device__ __forceinline__ T tx_corners(T* F, const int32_t* id3, int nx, int ny, int nz, const T* w8, T* rec,
                                         const T* src, bool inj, bool live, int q, int g0) {
-    int I = 0, J = 0, K = 0;
-    if (live) { I = id3[0]; J = id3[1]; K = id3[2]; }
-    const bool ok = live && I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz;
-    const int c = (I - 1) + nx * ((J - 1) + ny * (K - 1)) + (q & 1) + nx * ((q >> 1) & 1) + nx * ny * (q >> 2);
-    const T f = ok ? F[c] : (T)0;
-    const T w = ok ? w8[q] : (T)0;
-    const T r0 = live ? *rec : (T)0;
-    const T sv = (ok && inj) ? *src : (T)0;
-    const T prod = w * f;
-    T r = r0;
-#pragma unroll
-    for (int qq = 0; qq < 8; ++qq) r = r + __shfl(prod, g0 + qq, 64);
-    if (!ok) r = r0;
-    if (ok && q == 0) *rec = r;
-    if (ok && inj) F[c] = f + w * sv;
-    return r;
+    return tx_lane_apply<T>(F, tx_lane_load<T>(id3, nx, ny, nz, w8, rec, src, inj, live, q), rec, inj, q, g0);
 }
 
 // Pressure transceivers of one conflict-free level (forward.jl:92), 8 lanes
@@ -336,21 +358,43 @@ __device__ __forceinline__ void gf_hist_write(const GfHist& H, int is, int nt, i
 
 // One order_v entry e over the 32 lanes of a wave half (lw: lane in the
 // wave; channel x, y, z on lanes 0-7, 8-15, 16-23 of the half; `any`: the
-// entry is real).  All 32 lanes must call this (shuffles).
+// entry is real).  All 32 lanes must call this (shuffles).  tx_v_load /
+// tx_v_apply split it as tx_lane_load / tx_lane_apply do.
 template <typename T>
-__device__ __forceinline__ void tx_v_entry(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, const Txs<T>& X, uint32_t e,
-                                           bool any, int lw, int it0, const GfHist& H) {
+struct TxV {
+    uint32_t e;
+    TxLane<T> L;
+};
+
+template <typename T>
+__device__ __forceinline__ TxV<T> tx_v_load(int nx, int ny, int nz, const Txs<T>& X, uint32_t e, bool any, int lw,
+                                            int it0) {
+    const int ch = (lw >> 3) & 3, q = lw & 7;
+    const int is = (int)(e & TX_INDEX);
+    const size_t o = (size_t)is * X.nt + it0;
+    const size_t cs = (size_t)X.ntx * X.nt;  // channel stride of src/rec
+    const int c1 = (ch < 3 ? ch : 2) + 1;
+    TxV<T> v;
+    v.e = e;
+    v.L = tx_lane_load<T>(X.idx + 3 * (is + X.ntx * c1), nx, ny, nz, X.w + 8 * (is + X.ntx * c1),
+                          X.rec + o + cs * c1, X.src + o + cs * c1, (e & TX_INJ) != 0, any && ch < 3, q);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ void tx_v_apply(T* Vx, T* Vy, T* Vz, const Txs<T>& X, const TxV<T>& v, bool any, int lw,
+                                           int it0, const GfHist& H) {
     const int ch = (lw >> 3) & 3, q = lw & 7, g32 = lw & ~31, g0 = lw & ~7;
     const bool live = any && ch < 3;
+    const uint32_t e = v.e;
     const int is = (int)(e & TX_INDEX);
     const bool inj = (e & TX_INJ) != 0, gf_out = (e & TX_GF_OUT) != 0;
     const int ntx = X.ntx, nt = X.nt;
     const size_t o = (size_t)is * nt + it0;
-    const size_t cs = (size_t)ntx * nt;  // channel stride of src/rec
+    const size_t cs = (size_t)ntx * nt;
     const int c1 = (ch < 3 ? ch : 2) + 1;
     T* F = c1 == 1 ? Vx : (c1 == 2 ? Vy : Vz);
-    const T r = tx_corners<T>(F, X.idx + 3 * (is + ntx * c1), nx, ny, nz, X.w + 8 * (is + ntx * c1),
-                              X.rec + o + cs * c1, X.src + o + cs * c1, inj, live, q, g0);
+    const T r = tx_lane_apply<T>(F, v.L, X.rec + o + cs * c1, inj, q, g0);
     const T rx = __shfl(r, g32, 64), ry = __shfl(r, g32 + 8, 64), rz = __shfl(r, g32 + 16, 64);
     if (!any) return;
     const T* d = X.dir + 4 * is;
@@ -363,6 +407,12 @@ __device__ __forceinline__ void tx_v_entry(T* Vx, T* Vy, T* Vz, int nx, int ny, 
         const T vn = X.vn_src[o + 1];
         X.src[o + 1 + cs * c1] = X.src[o + 1 + cs * c1] + vn * d[c1];
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void tx_v_entry(T* Vx, T* Vy, T* Vz, int nx, int ny, int nz, const Txs<T>& X, uint32_t e,
+                                           bool any, int lw, int it0, const GfHist& H) {
+    tx_v_apply<T>(Vx, Vy, Vz, X, tx_v_load<T>(nx, ny, nz, X, e, any, lw, it0), any, lw, it0, H);
 }
 
 // Velocity transceivers of one level (forward.jl:111-115), then v2vn!
