@@ -285,7 +285,8 @@ extern "C" int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t in
 
 extern "C" int adf_solver_set_option(adf_solver* s, int option, int64_t value) {
     if (!s) return fail(ADF_EINVAL, "adf_solver_set_option: null solver");
-    touch(s);
+    // options that change the tiling invalidate the tile lists themselves;
+    // timing, store hint and transceiver launch form leave them valid
     return solver_set_option(s, option, value);
 }
 

@@ -20,6 +20,15 @@ int fail(int code, const std::string& msg);
             return adf::fail(ADF_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
+// Waits for event e by polling it, so that a run returns as soon as its
+// last step completes rather than when a sleeping thread is woken.
+inline hipError_t wait_event(hipEvent_t e) {
+    hipError_t r;
+    while ((r = hipEventQuery(e)) == hipErrorNotReady) {
+    }
+    return r;
+}
+
 struct adf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;

@@ -122,27 +122,48 @@ def read_traffic(name):
     return float(m.group(1)) if m else None
 
 
+def sweep_sample_period(steps):
+    """Every how many steps of a timed region the sweep is bracketed by HIP
+    events: every 16th step on long regions, and at least 5 samples (at most
+    every step) on short ones."""
+    return max(1, min(SWEEP_SAMPLE, steps // 5))
+
+
+def plan_steps(args):
+    """Step ranges of the 256^3 legs: the domain's step count and, per leg,
+    (warm-up first, warm-up last, timed first, timed last), 1-based and
+    inclusive like adf_solver_run.  Every range lies inside 1..nt."""
+    legs = {}
+    for name, steps in (("f32", args.steps), ("f64", args.fp64_steps)):
+        if steps > 0:
+            legs[name] = (1, args.warmup, args.warmup + 1, args.warmup + steps)
+    nt = max([r[3] for r in legs.values()] + [1]) + 1
+    return nt, legs
+
+
 def gpu_leg(args):
     import adfdtd as A
     from adfdtd import _lib as L
 
-    nt = args.warmup + args.steps
-    d, txs, p = build_workload(nt=nt + 1)
+    nt, legs = plan_steps(args)
+    d, txs, p = build_workload(nt=nt)
     out = {}
-    for name, dtype, prec, steps in (("f32", np.float32, "fast", args.steps), ("f64", np.float64, "fast", args.fp64_steps)):
-        if steps <= 0:
+    for name, dtype, prec in (("f32", np.float32, "fast"), ("f64", np.float64, "fast")):
+        if name not in legs:
             continue
+        w0, w1, t_first, t_last = legs[name]
+        steps = t_last - t_first + 1
         cp = A.make_cpml(d, p, 1e-4, FC, dtype)
         s = A.Solver(d, A.dtype_code(dtype, prec), cp)
         s.build_mask("sphere", [24 * d.dx])
         shape = A.tune_solver(s)
         s.set_transceivers(A.transceivers_to_grid(d, txs, dtype))
-        s.run(1, args.warmup)
-        # events around the sweep of every SWEEP_SAMPLE-th step of the timed
-        # region (an event pair adds ~18 us to its step at 256^3)
-        s.set_option(L.ADF_OPT_TIME_SWEEP, SWEEP_SAMPLE)
+        if w1 >= w0:
+            s.run(w0, w1)
+        # events around the sweep of sampled steps of the timed region
+        s.set_option(L.ADF_OPT_TIME_SWEEP, sweep_sample_period(steps))
         t0 = time.perf_counter()
-        s.run(args.warmup + 1, args.warmup + steps)
+        s.run(t_first, t_last)
         wall = time.perf_counter() - t0
         sweep_ms, launches = s.sweep_stats()
         peak = float(np.abs(s.get_trace(L.ADF_REC_P)[1:]).max())
@@ -169,6 +190,43 @@ def gpu_leg(args):
     return d, p, out
 
 
+def host_cpu():
+    """The host's CPU model, the logical CPUs this process may run on, the
+    physical cores among them (distinct (package, core) pairs), and the
+    thread share the environment grants (OMP_NUM_THREADS, when set)."""
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            cores.add((open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return {"model": model, "logical_cpus": len(cpus), "physical_cores": len(cores), "thread_share": share or None}
+
+
+def cpu_threads(info):
+    """Every physical core the process may use, capped by the environment's
+    thread share when it sets one."""
+    n = info["physical_cores"]
+    return min(n, info["thread_share"]) if info["thread_share"] else n
+
+
+def cpu_sample_steps(nt, one_step_s, max_seconds):
+    """Steps of the CPU sample after two warm-up steps: about max_seconds of
+    work, at least 2, and within the domain's nt (steps 3..2+k)."""
+    return int(max(2, min(nt - 2, max_seconds / max(one_step_s, 1e-3))))
+
+
 def cpu_leg(d, txs, p, max_seconds=15.0):
     """The oracle in the reference's own Float32 arithmetic, OpenMP over k,
     on a bounded sample: consecutive steps of the same 256^3 workload."""
@@ -176,7 +234,8 @@ def cpu_leg(d, txs, p, max_seconds=15.0):
     import oracle as O  # the checker / CPU baseline only
     import adfdtd as A
 
-    threads = min(os.cpu_count() or 1, 16)
+    info = host_cpu()
+    threads = cpu_threads(info)
     O.set_threads(threads)
     rd = O.domain(d.tmax, d.xmax, d.ymax, d.zmax, d.nx, d.ny, d.nz, d.c0, d.r0, d.cf)
     g = A.transceivers_to_grid(d, txs, np.float32)
@@ -188,13 +247,14 @@ def cpu_leg(d, txs, p, max_seconds=15.0):
     t0 = time.perf_counter()
     s.run(2, 2)
     one = time.perf_counter() - t0
-    k = int(max(2, min(d.nt - 3, max_seconds / max(one, 1e-3))))
+    k = cpu_sample_steps(d.nt, one, max_seconds)
     t0 = time.perf_counter()
     s.run(3, 2 + k)
     el = time.perf_counter() - t0
     return {"value": d.nx * d.ny * d.nz * k / el / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
             "sample": f"{k} consecutive steps (3..{2 + k}) of the same 256^3 workload, oracle in the reference's "
-                      f"Float32 arithmetic, OpenMP over k, {el:.1f} s"}
+                      f"Float32 arithmetic, OpenMP over k on {threads} threads, {el:.1f} s",
+            "host": info}
 
 
 def calibrate_copy():
@@ -521,7 +581,7 @@ def served_leg(reqs, budget=200 << 30):
             "peak_GB": round(st["peak_bytes"] / 2 ** 30, 3), "failed": st["failed"]}
 
 
-def main():
+def parse_args(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
@@ -538,7 +598,11 @@ def main():
     ap.add_argument("--batch-solves", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=1200)
     ap.add_argument("--served-requests", type=int, default=48)
-    args = ap.parse_args()
+    return ap.parse_args(argv)
+
+
+def main():
+    args = parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
               "independent solves shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
@@ -588,7 +652,7 @@ def main():
                      "kernel": "k_sweep (field-update sweep of one step: one launch over interior and side tiles)",
                      "algorithmic_bytes_per_launch": b32,
                      "avg_launch_ms": round(f32["sweep_ms"], 5), "launches": f32["sweep_launches"],
-                     "timed_every": SWEEP_SAMPLE,
+                     "timed_every": sweep_sample_period(args.steps),
                      "tiling": f32["tiling"]},
     }
     io = g["f32_pcie"]
