@@ -105,6 +105,22 @@ def test_random_field_every_cell_active(kind, shape, size, path):
         assert all(v == 0.0 for v in errs.values()), errs
 
 
+@pytest.mark.parametrize("kind", ["f64", "ref", "fast"])
+def test_anisotropic_spacing(kind, path):
+    # dx != dy != dz (different xmax, ymax, zmax per axis) and nx != ny != nz:
+    # a mix-up of axes, spacings or CPML slabs shows here; random field so
+    # every cell and memory entry is active, an ellipsoid scatterer, and a
+    # source injecting into every channel
+    d, txs, cpml, mask = build(n=(34, 30, 27), h=(0.00686, 0.0081, 0.0059), nt=40, npml=5, shape="ellipsoid",
+                               shape_params=[0.03, 0.022, 0.026], n_rec=20, rec_radius_cells=6,
+                               src_cells=(1.3, -0.7, 0.4), src_dir=(1.0, 0.3, -0.4, 0.5))
+    assert len({d.dx, d.dy, d.dz}) == 3
+    f0 = np.random.default_rng(5).standard_normal((4, d.nz, d.ny, d.nx)) * 1e-3
+    errs = _compare(d, txs, cpml, mask, kind, field=f0 if kind == "f64" else f0.astype(np.float32), path=path)
+    if kind == "f64":
+        assert all(v == 0.0 for v in errs.values()), errs
+
+
 def test_overlapping_injectors_keep_sequential_order(path):
     # several transceivers inject into shared cells: the result depends on the
     # order of the transceiver loop (derivatives.jl:30-60); levels keep it.
