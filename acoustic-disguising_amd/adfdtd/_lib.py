@@ -32,7 +32,7 @@ EXPORTS = (
     "adf_server_create", "adf_server_destroy", "adf_server_set_option", "adf_server_submit", "adf_server_poll",
     "adf_server_wait", "adf_server_get_trace", "adf_server_get_field", "adf_server_request_error",
     "adf_server_release", "adf_server_stats", "adf_kh_analytic_accumulate", "adf_kh_last_ms",
-    "adf_solver_gf_resample", "adf_resample_f32",
+    "adf_solver_gf_resample", "adf_resample_f32", "adf_resample_f32_noalloc",
 )
 ADF_REQ_QUEUED, ADF_REQ_RUNNING, ADF_REQ_DONE, ADF_REQ_FAILED = 0, 1, 2, -1
 ADF_SERVER_CHUNK, ADF_SERVER_BUDGET = 1, 2
@@ -151,6 +151,7 @@ def load():
             "adf_kh_last_ms": ([C.POINTER(d)], i32),
             "adf_solver_gf_resample": ([p, i64, i64, d, i64, p, p], i32),
             "adf_resample_f32": ([p, p, i64, i64, d, d, d, d, i64, d, i32, p], i32),
+            "adf_resample_f32_noalloc": ([p, p, i64, i64, d, d, d, d, i64, d, i32, p], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

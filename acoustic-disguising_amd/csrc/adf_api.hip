@@ -48,6 +48,13 @@ extern "C" int adf_ctx_create(int device, adf_ctx** out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    const int rc = resample_scratch_init(device);
+    if (rc) {
+        hipStreamDestroy(c->side);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     *out = c;
     return ADF_OK;
 }

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -105,7 +106,45 @@ __global__ void k_rs_eval(const double* __restrict__ coef, int n, int ncol, doub
     out[g] = (float)(v * scale);
 }
 
+// The pivot sequence of the spline system, cp_0 = 1/4, cp_i = 1/(4 -
+// cp_{i-1}), i < m: one thread, the same divisions the host makes.
+__global__ void k_rs_pivots(double* __restrict__ cp, int m) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double c = 1.0 / 4.0;
+    cp[0] = c;
+    for (int i = 1; i < m; ++i) {
+        c = 1.0 / (4.0 - c);
+        cp[i] = c;
+    }
+}
+
+// Device scratch of adf_resample_f32_noalloc: the pivot sequence, then the
+// spline coefficients of every column.  32 MiB (4 Mi doubles) on every
+// device the library runs on, held until the process ends.  In host code
+// `scratch` names a host-side stand-in, not device memory: the host uses
+// only the address hipGetSymbolAddress returns for each device.
+constexpr int64_t RS_SCRATCH_DOUBLES = (int64_t)1 << 22;
+constexpr int RS_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[RS_SCRATCH_DOUBLES];
+std::mutex g_scratch_mu;
+double* g_scratch[RS_MAX_DEVICES] = {};
+
+// Doubles of scratch before the coefficients (the pivots, padded to 256 B).
+int64_t rs_pivot_span(int64_t n) { return (std::max<int64_t>(n - 2, 1) + 31) / 32 * 32; }
+
 }  // namespace
+
+// Looks up this device's copy of the resample scratch once; device is the
+// current device.  Called by adf_ctx_create.
+int adf::resample_scratch_init(int device) {
+    if (device < 0 || device >= RS_MAX_DEVICES) return fail(ADF_EINVAL, "resample scratch: device index too large");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return ADF_OK;
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[device] = (double*)p;
+    return ADF_OK;
+}
 
 // The spline resample of ncol columns (see k_rs_coef) on stream st into
 // host out (n_out samples per column, column after column).
@@ -181,4 +220,37 @@ extern "C" int adf_resample_f32(adf_ctx* ctx, const float* in, int64_t n_in, int
     HIPCHK(hipMemcpyAsync(din.p, in, (size_t)ncol * n_in * 4, hipMemcpyHostToDevice, st));
     return resample_columns<float>(st, (const float*)din.p, (const float*)din.p, 0, ncol, n_in, ncol, denoise ? 1 : 0,
                                    t_in0, dt_in, t_out0, dt_out, n_out, amp_scale, out);
+}
+
+extern "C" int adf_resample_f32_noalloc(adf_ctx* ctx, const float* in, int64_t n_in, int64_t ncol, double t_in0,
+                                        double dt_in, double t_out0, double dt_out, int64_t n_out, double amp_scale,
+                                        int denoise, float* out) {
+    if (!ctx) return fail(ADF_EINVAL, "adf_resample_f32_noalloc: null context");
+    if (!in || !out || n_in < 1 || ncol < 1 || n_out < 1)
+        return fail(ADF_EINVAL, "adf_resample_f32_noalloc: empty or missing input/output");
+    if (!(dt_in > 0.0) || !(dt_out > 0.0))
+        return fail(ADF_EINVAL, "adf_resample_f32_noalloc: dt_in and dt_out must be > 0");
+    if (ncol * n_out > ((int64_t)1 << 31) || ncol > (1 << 30) || n_in > (1 << 30))
+        return fail(ADF_EINVAL, "adf_resample_f32_noalloc: too many samples");
+    const int64_t span = rs_pivot_span(n_in);
+    if (span + ncol * (n_in + 2) > RS_SCRATCH_DOUBLES)
+        return fail(ADF_EINVAL, "adf_resample_f32_noalloc: ncol * (n_in + 2) + " + std::to_string(span) +
+                                    " doubles exceed the 32 MiB scratch (" + std::to_string(RS_SCRATCH_DOUBLES) +
+                                    " doubles)");
+    double* base = ctx->device >= 0 && ctx->device < RS_MAX_DEVICES ? g_scratch[ctx->device] : nullptr;
+    if (!base) return fail(ADF_EDEVICE, "adf_resample_f32_noalloc: scratch not set up for this device");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    double* cp = base;
+    double* coef = base + span;
+    // every pivot and coefficient the evaluation reads is written first:
+    // nothing in the scratch needs clearing
+    hipLaunchKernelGGL(k_rs_pivots, dim3(1), dim3(1), 0, st, cp, (int)std::max<int64_t>(n_in - 2, 1));
+    hipLaunchKernelGGL((k_rs_coef<float>), dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, st, in, in, (int64_t)0,
+                       (int)ncol, (int)n_in, (int)ncol, denoise ? 1 : 0, (const double*)cp, coef);
+    const int64_t tot = ncol * n_out;
+    hipLaunchKernelGGL(k_rs_eval, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, (const double*)coef,
+                       (int)n_in, (int)ncol, t_in0, dt_in, t_out0, dt_out, (int)n_out, amp_scale, out);
+    HIPCHK(hipGetLastError());
+    return ADF_OK;
 }

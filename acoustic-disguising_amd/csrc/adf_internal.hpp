@@ -214,6 +214,7 @@ int solver_set_gf(adf_solver* s, int kind, const double* const* tensors, int64_t
                   int64_t dim3, const int64_t* iRec, int64_t n_irec, const int64_t* iSrc, int64_t n_isrc,
                   int writeback);
 int solver_run(adf_solver* s, int64_t start_step, int64_t end_step);
+int resample_scratch_init(int device);
 int solver_slice(adf_solver* s, int axis, int64_t index, void* out);
 void* cur_field(adf_solver* s);
 void* cur_mv(adf_solver* s, int a);

@@ -315,6 +315,22 @@ int adf_solver_gf_resample(adf_solver* s, int64_t first, int64_t n_rec, double t
 int adf_resample_f32(adf_ctx* ctx, const float* in, int64_t n_in, int64_t ncol, double t_in0, double dt_in,
                      double t_out0, double dt_out, int64_t n_out, double amp_scale, int denoise, float* out);
 
+/* adf_resample_f32 for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  in and out are DEVICE
+ * pointers the caller owns (same layouts as adf_resample_f32); the pivot
+ * sequence and the spline coefficients live in a fixed 32 MiB device array
+ * of the library (one per device, held until the process ends), so
+ * (n_in - 2 rounded up to 32) + ncol * (n_in + 2) must not exceed 4 Mi
+ * doubles (ADF_EINVAL otherwise).  Calls no hipMalloc/hipFree and
+ * allocates no host memory; enqueues three kernels on ctx's stream and
+ * returns: out is complete once the stream reaches that point
+ * (adf_ctx_synchronize).  Every call on one device shares the array:
+ * calls that overlap on one device (two contexts, or two threads) race. */
+int adf_resample_f32_noalloc(adf_ctx* ctx, const float* in, int64_t n_in, int64_t ncol, double t_in0,
+                             double dt_in, double t_out0, double dt_out, int64_t n_out, double amp_scale,
+                             int denoise, float* out);
+
 /* --------------------------------------------------------- post-hoc KH
  *
  * Analytic Kirchhoff-Helmholtz kernels evaluated on the device and
