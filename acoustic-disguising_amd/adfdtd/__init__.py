@@ -19,5 +19,6 @@ from .autotune import tune_batch, tune_solver
 from .kh import kh_analytic_accumulate
 from .greens import impulsive_gfs, scatterer_params
 from .reverb import build_reverb_data, julia_step_range, resample_traces
+from .hologram import HologramMode, MDDMode, RealMode, RefMode, build_hologram, hologram_mode, slice_indices
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -32,7 +32,7 @@ EXPORTS = (
     "adf_server_create", "adf_server_destroy", "adf_server_set_option", "adf_server_submit", "adf_server_poll",
     "adf_server_wait", "adf_server_get_trace", "adf_server_get_field", "adf_server_request_error",
     "adf_server_release", "adf_server_stats", "adf_kh_analytic_accumulate", "adf_kh_last_ms",
-    "adf_solver_gf_resample", "adf_resample_f32", "adf_resample_f32_noalloc",
+    "adf_solver_gf_resample", "adf_resample_f32", "adf_resample_f32_noalloc", "adf_bandwidth_probe",
 )
 ADF_REQ_QUEUED, ADF_REQ_RUNNING, ADF_REQ_DONE, ADF_REQ_FAILED = 0, 1, 2, -1
 ADF_SERVER_CHUNK, ADF_SERVER_BUDGET = 1, 2
@@ -40,6 +40,7 @@ ADF_OPT_PATH, ADF_OPT_TILE_Y, ADF_OPT_BLOCK_Z, ADF_OPT_TIME_SWEEP, ADF_OPT_WAVES
 ADF_OPT_PIPELINE, ADF_OPT_SWEEP_LAUNCH, ADF_OPT_STORE_NT, ADF_OPT_TX_LAUNCH = 7, 8, 9, 10
 ADF_SWEEP_ONE_LAUNCH, ADF_SWEEP_TWO_STREAMS, ADF_SWEEP_TWO_LAUNCHES = 0, 1, 2
 ADF_BATCH_PERSISTENT, ADF_BATCH_GRAPH_STEPS = 1, 2
+ADF_BW_COPY_BUF, ADF_BW_COPY_GLOBAL, ADF_BW_READ, ADF_BW_WRITE, ADF_BW_COPY_NT = 0, 1, 2, 3, 4
 
 
 class AdfDomain(C.Structure):
@@ -152,6 +153,7 @@ def load():
             "adf_solver_gf_resample": ([p, i64, i64, d, i64, p, p], i32),
             "adf_resample_f32": ([p, p, i64, i64, d, d, d, d, i64, d, i32, p], i32),
             "adf_resample_f32_noalloc": ([p, p, i64, i64, d, d, d, d, i64, d, i32, p], i32),
+            "adf_bandwidth_probe": ([p, i32, i64, i32, p], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

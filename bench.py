@@ -19,7 +19,9 @@ Modes for profiling (see profiles/):
   --profile-batch N   run the 64 x 128^3 batched leg for N steps only
   --profile-gf        run the in-loop KH gather leg only
   --profile-hologram  run the 192^3 hologram synthesis leg only
-  --calibrate-copy    time a plain 1 GiB device copy (PMC scale factor)
+  --calibrate-copy    HBM ceilings of the library's streaming probe kernels
+                      (copy/read/write; --probe-kind K for one kind, the
+                      run the PMC scale factors are read from)
 """
 from __future__ import annotations
 
@@ -257,21 +259,31 @@ def cpu_leg(d, txs, p, max_seconds=15.0):
             "host": info}
 
 
-def calibrate_copy():
-    import torch
+def calibrate_copy(kinds=None, sizes=(1 << 28, 1 << 30), reps=20):
+    """HBM ceilings with the library's own streaming kernels
+    (adf_bandwidth_probe): copy with the sweep's 128-bit buffer loads and
+    stores, copy with global loads and stores, read only, write only, copy
+    with non-temporal stores; buffers of 256 MiB (the size of the GPU's
+    last-level cache) and 1 GiB."""
+    import ctypes as C
 
-    n = 1 << 28  # 1 GiB of float32
-    a = torch.ones(n, device="cuda", dtype=torch.float32)
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(10):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    el = (time.perf_counter() - t0) / 10
-    print(json.dumps({"copy_bytes_each_way": n * 4, "ms": el * 1e3, "GBps_read_plus_write": 2 * n * 4 / el / 1e9}))
+    from adfdtd import _lib as L
+    from adfdtd.solver import context
+
+    names = {L.ADF_BW_COPY_BUF: "copy_buffer", L.ADF_BW_COPY_GLOBAL: "copy_global", L.ADF_BW_READ: "read",
+             L.ADF_BW_WRITE: "write", L.ADF_BW_COPY_NT: "copy_nontemporal"}
+    lib = L.load()
+    ctx = context(0)
+    out = {}
+    for kind in kinds if kinds is not None else sorted(names):
+        for nb in sizes:
+            ms = C.c_double()
+            L.check(lib.adf_bandwidth_probe(ctx, kind, nb, reps, C.byref(ms)), "adf_bandwidth_probe")
+            moved = nb * (2 if kind in (L.ADF_BW_COPY_BUF, L.ADF_BW_COPY_GLOBAL, L.ADF_BW_COPY_NT) else 1)
+            out[f"{names[kind]}_{nb >> 20}MiB"] = {"ms": round(ms.value, 4), "bytes_moved": moved,
+                                                   "GBps": round(moved / (ms.value * 1e-3) / 1e9, 1),
+                                                   "frac_of_peak": round(moved / (ms.value * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    print(json.dumps(out))
 
 
 def batch_bytes(specs):
@@ -593,6 +605,9 @@ def parse_args(argv=None):
     ap.add_argument("--profile-gf", action="store_true")
     ap.add_argument("--profile-hologram", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--probe-kind", type=int, default=-1, help="with --calibrate-copy: one probe kind only")
+    ap.add_argument("--probe-bytes", type=int, default=1 << 30)
+    ap.add_argument("--probe-reps", type=int, default=20)
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-served", action="store_true")
     ap.add_argument("--batch-solves", type=int, default=64)
@@ -608,7 +623,10 @@ def main():
               "independent solves shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
-        calibrate_copy()
+        if args.probe_kind >= 0:
+            calibrate_copy([args.probe_kind], (args.probe_bytes,), args.probe_reps)
+        else:
+            calibrate_copy()
         return
     if args.profile_gf:
         print(json.dumps(gf_map_leg()))

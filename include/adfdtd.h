@@ -356,6 +356,21 @@ int adf_kh_analytic_accumulate(adf_ctx* ctx, int which, int dtype, int64_t n_out
  * thread's last adf_kh_analytic_accumulate (measurement support). */
 int adf_kh_last_ms(double* ms);
 
+/* ------------------------------------------------- bandwidth probes
+ *
+ * Measurement support (no counterpart in the reference): streaming kernels
+ * over two device buffers of `bytes` each (a multiple of 16 KiB, at most
+ * 1 GiB) with the 128-bit access instructions of the field-update sweep;
+ * *ms_per_launch is the mean device time of `reps` launches after one
+ * warm-up launch.  A copy moves 2 * bytes per launch, read and write
+ * kinds `bytes`. */
+#define ADF_BW_COPY_BUF 0    /* copy, buffer loads and stores */
+#define ADF_BW_COPY_GLOBAL 1 /* copy, global loads and stores */
+#define ADF_BW_READ 2        /* read only */
+#define ADF_BW_WRITE 3       /* write only */
+#define ADF_BW_COPY_NT 4     /* copy, non-temporal stores */
+int adf_bandwidth_probe(adf_ctx* ctx, int kind, int64_t bytes, int reps, double* ms_per_launch);
+
 /* ----------------------------------------------------------------- server
  *
  * Served form (an addition): a request loop on its own host thread and HIP
