@@ -19,6 +19,8 @@ from .autotune import tune_batch, tune_solver
 from .kh import kh_analytic_accumulate
 from .greens import impulsive_gfs, scatterer_params
 from .reverb import build_reverb_data, julia_step_range, resample_traces
+from .greens_io import (fib_sphere_area, gf_to_physical, gf_to_raw_fdtd, gf_unit_factors, load_gf_mdd, load_gf_ref,
+                        read_gf_file, scaling_factors, write_gf_file)
 from .hologram import HologramMode, MDDMode, RealMode, RefMode, build_hologram, hologram_mode, slice_indices
 
 __all__ = [n for n in dir() if not n.startswith("_")]
