@@ -20,11 +20,12 @@
 // exp(-50) of their peak; erf (A&S 7.1.26, kh_kernels.jl:19-25) rounds to
 // exactly +-1 there.
 //
-// One workgroup = one receiver s and 256 consecutive output times; it
-// walks the surface points r in order, so out[t, s] accumulates the r
-// contributions in the reference's order.
+// The convolution runs on the VALUs as register-blocked FMAs (k_kh_conv
+// below); the surface points are split into chunks whose partial totals
+// are summed in chunk order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -36,8 +37,6 @@ using namespace adf;
 
 namespace {
 
-constexpr int KH_T = 256;    // output times per workgroup
-constexpr int KH_W = 1024;   // kernel samples staged in LDS per chunk
 constexpr double KH_HALF_WIDTH = 10.0;  // window half-width in sigma
 
 __device__ __forceinline__ double kh_erf(double x) {  // kh_kernels.jl:19-25
@@ -110,103 +109,194 @@ __global__ void k_kh_prefix(const T* in, int nt, int no, double* cum) {
     }
 }
 
-// T: input/kernel-sample type (float: K rounded to Float32 as the
-// reference's eval_one_kernel(; T = Float32), products in float, sums in
-// double); PED: pedestal form.
+// The convolution, register-blocked.  One wave = one receiver s, 64 R
+// consecutive output times and a chunk of surface points r.  Lane l owns
+// the R consecutive outputs t = T0 + R l + j; for each r the window [a, b]
+// is taken in chunks of taps tt = c1 - kk, kk = 0..16R-1: the kernel samples
+// Kw[kk] (wave-uniform, read from LDS as broadcasts) and the inputs
+// Is[i] = in[T0 - c1 + 1 + i, r] (zero outside 1..nt) are staged in LDS,
+// and output j of lane l takes Kw[kk] * Is[R l + j + kk].  The inputs a
+// lane needs for R taps lie in two R-long segments of Is: the segments
+// alternate as the taps advance, so each group of R taps loads one new
+// segment (R LDS reads for R^2 multiply-adds) and its R kernel samples as
+// 16-byte broadcasts.  R is odd, so the 64 lanes'
+// segments start in distinct LDS banks.  R = 17 for float and double: the
+// register budget of two waves per SIMD.
+//
+// T = float: kernel samples rounded to Float32 (eval_one_kernel(; T =
+// Float32)), inputs Float32, products accumulated with Float32 FMAs over
+// one (r, s) window and added to a double total; T = double: all double.
+// Zero-padded inputs take care of tt > t (u < 1).  Partial totals of the
+// chunks of r are summed in chunk order afterwards (k_kh_reduce), so the
+// result does not depend on scheduling.
+#ifndef ADF_KH_R32
+#define ADF_KH_R32 17  // outputs per lane for float (odd); 17 keeps two waves per SIMD
+#endif
+template <typename T>
+struct KC {
+    static constexpr int R = sizeof(T) == 4 ? ADF_KH_R32 : 17;
+    static constexpr int OUT = 64 * R;
+    static constexpr int W = 16 * R;  // taps per LDS chunk (a multiple of 2 R): 272, one window of 20 sigma / dt
+    // kernel samples are stored in groups of R taps at a stride of KS floats
+    // (16-byte aligned groups: broadcast reads of four taps at once)
+    static constexpr int KS = (R + 3) / 4 * 4;
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void kc_seg(const T* Is, int base, T (&x)[R]) {
+#pragma unroll
+    for (int m = 0; m < R; ++m) x[m] = Is[base + m];
+}
+
+// The R taps of group g (kk = gR .. gR + R - 1, stored at Kg = Kw + g KS):
+// A = Is[R l + gR ..], B = the next segment.
+template <typename T, int R>
+__device__ __forceinline__ void kc_block(const T* __restrict__ Kg, const T (&A)[R], const T (&B)[R], T (&acc)[R]) {
+    constexpr int KS = (R + 3) / 4 * 4;
+    T kv[KS];
+    if constexpr (sizeof(T) == 4) {
+        typedef float F4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int q = 0; q < KS; q += 4) {
+            const F4 v = *reinterpret_cast<const F4*>(Kg + q);
+            kv[q] = v[0], kv[q + 1] = v[1], kv[q + 2] = v[2], kv[q + 3] = v[3];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < R; ++q) kv[q] = Kg[q];
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = fma(kv[q], j + q < R ? A[j + q] : B[j + q - R], acc[j]);
+    }
+}
+
 template <typename T, int WHICH, bool PED>
-__global__ __launch_bounds__(KH_T) void k_kh_direct(const double* __restrict__ op, const double* __restrict__ on,
-                                                    int no, const double* __restrict__ ip,
-                                                    const double* __restrict__ inrm, int ni,
-                                                    const double* __restrict__ tg, int ntK, double sg, double c0,
-                                                    double rho0, double alpha, const T* __restrict__ in, int nt,
-                                                    const double* __restrict__ cum, double* __restrict__ out) {
-    __shared__ T Ks[KH_W];
-    __shared__ T Is[KH_W + KH_T];
+__global__ __launch_bounds__(64) void k_kh_conv(const double* __restrict__ op, const double* __restrict__ on, int no,
+                                                const double* __restrict__ ip, const double* __restrict__ inrm, int ni,
+                                                const double* __restrict__ tg, int ntK, double sg, double c0,
+                                                double rho0, double alpha, const T* __restrict__ in, int nt,
+                                                const double* __restrict__ cum, int rpc, double* __restrict__ dst) {
+    constexpr int R = KC<T>::R, OUT = KC<T>::OUT, KW = KC<T>::W, KS = KC<T>::KS;
+    __shared__ __attribute__((aligned(16))) T Kw[KW / R * KS];
+    __shared__ T Is[OUT + KW];
+    const int lane = (int)threadIdx.x;
     const int s = blockIdx.y;
-    const int T0 = blockIdx.x * KH_T + 1;  // first output time of the tile (1-based)
-    const int t = T0 + (int)threadIdx.x;
-    const bool live = t <= nt;
-    const int m = live ? min(t, ntK) : 0;  // tt runs over 1..m
-    double total = 0.0;
-    for (int r = 0; r < no; ++r) {
+    const int T0 = blockIdx.z * OUT + 1;  // first output time of the wave (1-based)
+    const int r0 = blockIdx.x * rpc, r1 = min(no, r0 + rpc);
+    const int tl = T0 + R * lane;         // the lane's first output
+    double tot[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) tot[j] = 0.0;
+    for (int r = r0; r < r1; ++r) {
         const Pair P = pair_of(op, on, no, r, ip, inrm, ni, s, c0);
         // window [a, b] (1-based, clipped to 1..ntK)
         const int a = count_below(tg, ntK, P.retard - KH_HALF_WIDTH * sg, false) + 1;
         const int b = count_below(tg, ntK, P.retard + KH_HALF_WIDTH * sg, true);
         const double kc = (P.cr * P.cs / rho0) * (2 * P.i3);  // the F = 1 level of v_v; also K_dc
-        double acc = 0.0;
-        // ---- window part, in chunks of KH_W kernel samples
-        for (int c0i = a; c0i <= b; c0i += KH_W) {
-            const int c1 = min(b, c0i + KH_W - 1);
-            const int w = c1 - c0i + 1;
+        const T* inr = in + (size_t)r * nt;
+        T acc[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = (T)0;
+        for (int c1 = b; c1 >= a; c1 -= KW) {
+            const int cl = max(a, c1 - KW + 1);
+            const int w = c1 - cl + 1;
+            const int wp = (w + 2 * R - 1) / (2 * R) * (2 * R);  // taps rounded to whole pairs of groups
             __syncthreads();
-            for (int j = threadIdx.x; j < w; j += KH_T) {
-                double k = kernel_at<WHICH>(P, tg[c0i + j - 1] - P.retard, sg, c0, rho0);
-                if constexpr (PED && WHICH == 3) k = (double)(T)k - (double)(T)kc;
-                Ks[j] = (T)k;
+            for (int kk = lane; kk < wp; kk += 64) {
+                T kv = (T)0;
+                if (kk < w) {
+                    double k = kernel_at<WHICH>(P, tg[c1 - kk - 1] - P.retard, sg, c0, rho0);
+                    if constexpr (PED && WHICH == 3) k = (double)(T)k - (double)(T)kc;
+                    kv = (T)k;
+                }
+                Kw[(kk / R) * KS + kk % R] = kv;
             }
-            // inputs u = t - tt + 1 for t in the tile and tt in [c0i, c1]:
-            // u in [T0 - c1 + 1, T0 + KH_T - 1 - c0i + 1]
-            const int u0 = T0 - c1 + 1;
-            const int nu = KH_T + w - 1;
-            for (int j = threadIdx.x; j < nu; j += KH_T) {
-                const int u = u0 + j;
-                Is[j] = (u >= 1 && u <= nt) ? in[(size_t)r * nt + (u - 1)] : (T)0;
+            const int ub = T0 - c1 + 1;  // u of Is[0]
+            for (int i = lane; i < OUT + wp; i += 64) {
+                const int u = ub + i;
+                Is[i] = (u >= 1 && u <= nt) ? inr[u - 1] : (T)0;
             }
             __syncthreads();
-            if (live) {
-                const int hi = min(c1, m);
-                for (int tt = c0i; tt <= hi; ++tt) {
-                    const T prod = Ks[tt - c0i] * Is[t - tt + 1 - u0];
-                    acc += (double)prod;
+            // group g reads segments g and g+1 (Is[base + gR .. base + gR +
+            // 2R - 1]); the last segment read, wp / R, ends at OUT + wp - 1
+            T A[R], B[R];
+            const int base = R * lane;
+            kc_seg<T, R>(Is, base, A);
+            kc_seg<T, R>(Is, base + R, B);
+            for (int g = 0; g < wp / R; g += 2) {
+                kc_block<T, R>(Kw + g * KS, A, B, acc);
+                kc_seg<T, R>(Is, base + (g + 2) * R, A);
+                kc_block<T, R>(Kw + (g + 1) * KS, B, A, acc);
+                if ((g + 2) * R < wp) kc_seg<T, R>(Is, base + (g + 3) * R, B);
+            }
+        }
+        const double* cr_ = cum + (size_t)r * (nt + 1);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int t = tl + j;
+            double v = (double)acc[j];
+            if (t <= nt) {
+                const int m = min(t, ntK);
+                auto csum = [&](int lo, int hi) {  // sum of in[u], u in [lo, hi] (1-based), clipped
+                    lo = max(lo, 1);
+                    hi = min(hi, nt);
+                    return hi >= lo ? cr_[hi] - cr_[lo - 1] : 0.0;
+                };
+                if constexpr (WHICH == 3 && !PED) {
+                    // after the window (tt in [b+1, m]) the kernel is the F = 1 level
+                    if (m > b) v += (double)(T)kc * csum(t - m + 1, t - max(b, 0));
+                }
+                if constexpr (WHICH == 3 && PED) {
+                    // before the window the residual K - K_dc is -K_dc (F = 0)
+                    const int e = min(a - 1, m);
+                    if (e >= 1) v += -(double)(T)kc * csum(t - e + 1, t);
                 }
             }
-        }
-        if (live && m >= 1) {
-            const double* cr_ = cum + (size_t)r * (nt + 1);
-            auto csum = [&](int lo, int hi) {  // sum of in[u], u in [lo, hi] (1-based), clipped
-                lo = max(lo, 1);
-                hi = min(hi, nt);
-                return hi >= lo ? cr_[hi] - cr_[lo - 1] : 0.0;
-            };
-            if constexpr (WHICH == 3 && !PED) {
-                // after the window (tt in [b+1, m]) the kernel is the F = 1 level
-                if (m > b) acc += (double)(T)kc * csum(t - m + 1, t - max(b, 0));
-            }
+            tot[j] += alpha * v;
             if constexpr (WHICH == 3 && PED) {
-                // before the window the residual K - K_dc is -K_dc (F = 0)
-                const int e = min(a - 1, m);
-                if (e >= 1) acc += -(double)(T)kc * csum(t - e + 1, t);
+                // + alpha K_dc[r, s] cumsum(in)[t, r] (kh_kernels.jl:285-294)
+                const double kd = (double)(T)kc;
+                if (t <= nt && kd != 0) tot[j] += (alpha * kd) * cr_[t];
             }
         }
-        total += alpha * acc;
     }
-    if constexpr (WHICH == 3 && PED) {
-        // + alpha K_dc[r, s] cumsum(in)[t, r] (kh_kernels.jl:285-294)
-        if (live)
-            for (int r = 0; r < no; ++r) {
-                const Pair P = pair_of(op, on, no, r, ip, inrm, ni, s, c0);
-                const double kc = (double)(T)((P.cr * P.cs / rho0) * (2 * P.i3));
-                if (kc == 0) continue;
-                total += (alpha * kc) * cum[(size_t)r * (nt + 1) + t];
-            }
+    // dst: the chunk's partial totals [chunk][s][t] (or out itself when
+    // there is one chunk: accumulate)
+    double* d = dst + ((size_t)blockIdx.x * ni + s) * nt;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int t = tl + j;
+        if (t <= nt) {
+            if (gridDim.x == 1) d[t - 1] += tot[j];
+            else d[t - 1] = tot[j];
+        }
     }
-    if (live) out[(size_t)s * nt + (t - 1)] += total;
+}
+
+// out[s][t] += sum over chunks c (in order) of part[c][s][t]
+__global__ void k_kh_reduce(const double* __restrict__ part, int nc, int64_t n, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = 0.0;
+    for (int c = 0; c < nc; ++c) v += part[(size_t)c * n + i];
+    out[i] += v;
 }
 
 template <typename T, int WHICH, bool PED>
 void launch(dim3 g, hipStream_t st, const double* op, const double* on, int no, const double* ip,
             const double* inn, int ni, const double* tg, int ntK, double sg, double c0, double rho0, double alpha,
-            const T* in, int nt, const double* cum, double* out) {
-    hipLaunchKernelGGL((k_kh_direct<T, WHICH, PED>), g, dim3(KH_T), 0, st, op, on, no, ip, inn, ni, tg, ntK, sg, c0,
-                       rho0, alpha, in, nt, cum, out);
+            const T* in, int nt, const double* cum, int rpc, double* dst) {
+    hipLaunchKernelGGL((k_kh_conv<T, WHICH, PED>), g, dim3(64), 0, st, op, on, no, ip, inn, ni, tg, ntK, sg, c0,
+                       rho0, alpha, in, nt, cum, rpc, dst);
 }
 
 template <typename T>
 void launch_which(int which, bool ped, dim3 g, hipStream_t st, const double* op, const double* on, int no,
                   const double* ip, const double* inn, int ni, const double* tg, int ntK, double sg, double c0,
-                  double rho0, double alpha, const T* in, int nt, const double* cum, double* out) {
-#define KH_L(W, P) launch<T, W, P>(g, st, op, on, no, ip, inn, ni, tg, ntK, sg, c0, rho0, alpha, in, nt, cum, out)
+                  double rho0, double alpha, const T* in, int nt, const double* cum, int rpc, double* dst) {
+#define KH_L(W, P) launch<T, W, P>(g, st, op, on, no, ip, inn, ni, tg, ntK, sg, c0, rho0, alpha, in, nt, cum, rpc, dst)
     if (which == 0) KH_L(0, false);
     else if (which == 1) KH_L(1, false);
     else if (which == 2) KH_L(2, false);
@@ -270,7 +360,19 @@ extern "C" int adf_kh_analytic_accumulate(adf_ctx* ctx, int which, int dtype, in
     }
     HIPCHK(hipMemcpyAsync(dout.p, out, (size_t)n_inner * nt * 8, hipMemcpyHostToDevice, st));
     const int no = (int)n_outer, ni = (int)n_inner, nti = (int)nt, nk = (int)ntK;
-    const dim3 g((unsigned)((nt + KH_T - 1) / KH_T), (unsigned)n_inner);
+    // waves: (chunk of surface points, receiver, block of 64 R output
+    // times); the points are split into chunks until there are about 4096
+    // waves
+    const int kc_out = f64 ? KC<double>::OUT : KC<float>::OUT;
+    const int nblk = (int)((nt + kc_out - 1) / kc_out);
+    const int64_t base_waves = (int64_t)ni * nblk;
+    int nc = (int)std::min<int64_t>(no, std::max<int64_t>(1, (4096 + base_waves - 1) / base_waves));
+    const int rpc = (no + nc - 1) / nc;
+    nc = (no + rpc - 1) / rpc;
+    DevBuf dpart;
+    if (nc > 1) HIPCHK(hipMalloc(&dpart.p, (size_t)nc * n_inner * nt * 8));
+    double* dst = nc > 1 ? (double*)dpart.p : (double*)dout.p;
+    const dim3 g((unsigned)nc, (unsigned)n_inner, (unsigned)nblk);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -280,13 +382,18 @@ extern "C" int adf_kh_analytic_accumulate(adf_ctx* ctx, int which, int dtype, in
                            no, (double*)dcum.p);
         launch_which<double>(which, pedestal != 0, g, st, (const double*)dop.p, (const double*)don.p, no,
                              (const double*)dip.p, (const double*)dinn.p, ni, (const double*)dtg.p, nk, sigma_t, c0,
-                             rho0, alpha, (const double*)din.p, nti, (const double*)dcum.p, (double*)dout.p);
+                             rho0, alpha, (const double*)din.p, nti, (const double*)dcum.p, rpc, dst);
     } else {
         hipLaunchKernelGGL((k_kh_prefix<float>), dim3((no + 255) / 256), dim3(256), 0, st, (const float*)din.p, nti,
                            no, (double*)dcum.p);
         launch_which<float>(which, pedestal != 0, g, st, (const double*)dop.p, (const double*)don.p, no,
                             (const double*)dip.p, (const double*)dinn.p, ni, (const double*)dtg.p, nk, sigma_t, c0,
-                            rho0, alpha, (const float*)din.p, nti, (const double*)dcum.p, (double*)dout.p);
+                            rho0, alpha, (const float*)din.p, nti, (const double*)dcum.p, rpc, dst);
+    }
+    if (nc > 1) {
+        const int64_t n = (int64_t)n_inner * nt;
+        hipLaunchKernelGGL(k_kh_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const double*)dpart.p,
+                           nc, n, (double*)dout.p);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, st));
