@@ -255,7 +255,9 @@ struct SPlane : IPlane<T, CPL, TY> {
 // zero-reading null resource) outside the slab, which leaves the derivative
 // unchanged, as the reference's slab test does (updates.jl:21-103,192-275,
 // 320-378,380-501).  The x slab is handled the same way per lane.
-template <typename T, bool EXACT, int CPL, int TY, bool SY, bool SZ, int POL = 0>
+// NOMASK: the instantiation for tiles that meet no masked cell (no mask
+// loads or registers).
+template <typename T, bool EXACT, int CPL, int TY, bool SY, bool SZ, int POL = 0, bool NOMASK = false>
 __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0, int k1, bool masked) {
     typedef FA<T, EXACT> M_;
     typedef typename M_::A A;
@@ -276,7 +278,7 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
     // null resources (every access out of range) for the disabled streams
     const Rsrc rSP0 = rsrc_of(F.psi0, cpP ? psib : 0u), rSV0 = rsrc_of(F.psi0, cpV ? psib : 0u);
     const Rsrc rM = rsrc_of(F.mask, masked ? (uint32_t)(4 * F.nwx * ny * nz) * 4u : 0u);
-    const uint32_t mfill = masked ? 0u : 0xffffffffu;  // all open without a mask
+    const uint32_t mfill = (masked && !NOMASK) ? 0u : 0xffffffffu;  // all open without a mask
     const int vl = i0c * ES;
     const int vs = lane_ok ? i0 * ES : (int)OOR;
     // per-channel lane offsets (the channel's start in the vector offset,
@@ -365,20 +367,32 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
             bld<T, CPL>(rF0, vlc[0], rb, L.p0[r]);
             bld<T, CPL>(rF0, vlc[1], rb, L.vx0[r]);
             bld<T, CPL>(rF0, vlc[3], rowb(j, k + 1), L.vzn[r]);
-            L.mp_[r] = bld_u32(rM, vmc[0], maskb(0, j, k));
             bld<T, CPL>(rSP0, vP, p * (j + ny * k) * ES, L.mPx[r]);
             if constexpr (SY)
                 bld<T, CPL>(rsrc_of(F.psi0, yinP[r] ? psib : 0u), vl, yoP[r] + k * ypP, L.mPy[r]);
             if constexpr (SZ) bld<T, CPL>(rsrc_of(F.psi0, L.zinP ? psib : 0u), vl, L.zoP + r * nx * ES, L.mPz[r]);
             if (r > 0) {
-                L.mvx_[r] = bld_u32(rM, vmc[1], maskb(0, j, k));
-                L.mvy_[r] = bld_u32(rM, vmc[2], maskb(0, j, k));
-                L.mvz_[r] = bld_u32(rM, vmc[3], maskb(0, j, k));
                 bld<T, CPL>(rSV0, vV, qv * (j + ny * k) * ES, L.mVx[r]);
                 if constexpr (SY)
                     bld<T, CPL>(rsrc_of(F.psi0, yinV[r] ? psib : 0u), vl, yoV[r] + k * ypV, L.mVy[r]);
                 if constexpr (SZ)
                     bld<T, CPL>(rsrc_of(F.psi0, L.zinV ? psib : 0u), vl, L.zoV + r * nx * ES, L.mVz[r]);
+            }
+        }
+        // update masks (a tile instantiated for unmasked tiles reads none:
+        // mfill opens every cell)
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) {
+            const int j = j0 + r - 1;
+            if constexpr (NOMASK) {
+                L.mp_[r] = L.mvx_[r] = L.mvy_[r] = L.mvz_[r] = 0u;
+                continue;
+            }
+            L.mp_[r] = bld_u32(rM, vmc[0], maskb(0, j, k));
+            if (r > 0) {
+                L.mvx_[r] = bld_u32(rM, vmc[1], maskb(0, j, k));
+                L.mvy_[r] = bld_u32(rM, vmc[2], maskb(0, j, k));
+                L.mvz_[r] = bld_u32(rM, vmc[3], maskb(0, j, k));
             }
         }
     };
@@ -485,13 +499,17 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
     // the scheduling barriers keep each prefetch ahead of the computation
     // it overlaps (the scheduler would otherwise sink the loads to shorten
     // register lifetimes)
+    // No prefetch after the last plane, except where registers are at
+    // the limit (the branch costs a few): there the last prefetch repeats
+    // the last plane.
+    constexpr bool SKIP_TAIL = ES == 4 || CPL < 4 || NOMASK;
     int k = k0;
     for (;;) {
-        load(B0, min(k + 1, k1 - 1));  // the last prefetch repeats a plane
+        if (!SKIP_TAIL || k + 1 < k1) load(B0, min(k + 1, k1 - 1));
         __builtin_amdgcn_sched_barrier(0);
         step(B1, k);
         if (++k >= k1) break;
-        load(B1, min(k + 1, k1 - 1));
+        if (!SKIP_TAIL || k + 1 < k1) load(B1, min(k + 1, k1 - 1));
         __builtin_amdgcn_sched_barrier(0);
         step(B0, k);
         if (++k >= k1) break;
@@ -517,8 +535,12 @@ __device__ __forceinline__ void sweep_tile(const Fused<T>& F, uint2 e) {
     const int kind = (int)((e.x >> 28) & 7u);
     constexpr int TI = sweep_ty_int((int)sizeof(T), CPL), TS = sweep_ty_side(CPL);
     if (kind == TILE_INT) {
-        if constexpr (PF) sweep_tile_pf<T, EXACT, CPL, TI, false, false, POL>(F, j0, k0, k1, masked);
-        else sweep_int_tile<T, EXACT, CPL, TI>(F, j0, k0, k1, masked);
+        if constexpr (PF) {
+            if (masked) sweep_tile_pf<T, EXACT, CPL, TI, false, false, POL>(F, j0, k0, k1, true);
+            else sweep_tile_pf<T, EXACT, CPL, TI, false, false, POL, true>(F, j0, k0, k1, false);
+        } else {
+            sweep_int_tile<T, EXACT, CPL, TI>(F, j0, k0, k1, masked);
+        }
     } else if (PF && kind == TILE_SY) {
         sweep_tile_pf<T, EXACT, CPL, TS, true, false, POL>(F, j0, k0, k1, masked);
     } else if (PF && kind == TILE_SZ) {
