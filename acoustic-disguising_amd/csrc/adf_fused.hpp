@@ -5,9 +5,9 @@
 // and writes the new ones (1), so a wave may recompute the pressure of a
 // neighbouring tile's edge cells (its halo) from untouched old values.
 //
-// One wave64 owns a whole x-row (nx <= 64*CPL; lane l holds the CPL
-// consecutive cells l*CPL.. as one vector load/store), TY rows in y held in
-// registers, and marches KB planes in z.  Per plane it loads every row of the
+// One wave64 owns an x-row segment (lane l holds CPL consecutive cells as
+// one vector load/store; see xlane), TY rows in y held in registers, and
+// marches KB planes in z.  Per plane it loads every row of the
 // tile plus the halo row below (j0-1), computes P1 on all of them, takes the
 // x-neighbours Vx0(i+1) and P1(i-1) from the adjacent lane with wave
 // shuffles, and writes P1, Vx1, Vy1, Vz1 of its own rows.  P1 of the previous
@@ -57,6 +57,7 @@ struct Fused {
     int psi_n;  // elements in psi0 / psi1
     int cpml_p, fuse_mirror;
     int ty;                        // rows per wave
+    int xseg;                      // owned cells per row segment (xlane)
     int off_mv[3], off_mp[3];      // element offsets of the CPML arrays in psi0/psi1
     double dx, dy, dz, fq, fv;
     float rdx, rdy, rdz, fqf, fvf;
@@ -174,6 +175,33 @@ __device__ __forceinline__ void bst(Rsrc r, int voff, int soff, const T (&o)[N])
         store_b128<POL>(u0, r, voff, soff);
         store_b128<POL>(u1, r, voff + 16, soff);
     }
+}
+
+// x placement of a lane.  A row of up to 64 CPL cells is one segment,
+// lane l holding cells l CPL..; a wider row is cut into segments of xseg =
+// 62 CPL owned cells, one wave each, whose lane 0 (after the first
+// segment) and lane 63 (before the last) hold the neighbouring segments'
+// edge cells: the wave computes their new pressure from the old buffers
+// like the owner does, so the x-neighbours Vx0(i+1) and P1(i-1) of its own
+// edge cells come from adjacent lanes as inside the row.  With nx not a
+// multiple of CPL the last lane of the row runs past its end (`part`): its
+// loads read cells of the next row (never used) and it stores cell by cell.
+struct XLane {
+    int i0;     // first cell of the lane
+    int i0c;    // first cell of its loads: i0 clamped into the row (a multiple of CPL)
+    bool own;   // the lane's cells belong to this segment (it stores them)
+    bool part;  // an owned lane whose cells run past the row end
+};
+template <int CPL>
+__device__ __forceinline__ XLane xlane(int nx, int xseg, int seg) {
+    const int lane = threadIdx.x & 63;
+    const int xs = seg * xseg;
+    XLane X;
+    X.i0 = (seg > 0 ? xs - CPL : 0) + lane * CPL;
+    X.own = X.i0 >= xs && X.i0 < min(xs + xseg, nx);
+    X.part = X.own && X.i0 + CPL > nx;
+    X.i0c = min(X.i0, (nx - 1) / CPL * CPL);
+    return X;
 }
 
 __device__ __forceinline__ uint32_t bld_u32(Rsrc r, int voff, int soff) {
@@ -355,16 +383,18 @@ struct FA {
 // planes, so only the x direction keeps special cases.  Loads use clamped
 // indices (always in bounds) instead of guards; values of rows, lanes and
 // planes outside the grid are computed but never stored.
-template <typename T, bool EXACT, int CPL, int TY, bool GEN, int POL = 0>
-__device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, int k1, bool masked) {
+// RAG: rows whose length is not a multiple of CPL (the partial lane stores
+// cell by cell); seg: the row segment (xlane).
+template <typename T, bool EXACT, int CPL, int TY, bool GEN, int POL = 0, bool RAG = false>
+__device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, int k1, bool masked, int seg) {
     typedef FA<T, EXACT> M_;
     typedef typename M_::A A;
     const int nx = F.nx, ny = F.ny, nz = F.nz, p = F.p, qv = p + 1;
     const int sxy = nx * ny;
-    const int lane = threadIdx.x & 63;
-    const int i0 = lane * CPL;
-    const bool lane_ok = i0 < nx;
-    const int i0c = lane_ok ? i0 : nx - CPL;  // clamped lane start (loads only)
+    const XLane XL = xlane<CPL>(nx, F.xseg, seg);
+    const int i0 = XL.i0;
+    const bool lane_in = i0 < nx;  // the lane has cells in the row (loads of CPML memory)
+    const int i0c = XL.i0c;        // clamped lane start (field and mask loads)
     const int ks = k0 > 0 ? k0 - 1 : k0;  // first plane (halo plane when k0 > 0)
     const uint32_t* Mk = masked ? F.mask : nullptr;  // tiles without closed cells skip the mask
     const int sh = i0c & 31;
@@ -380,13 +410,26 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, in
     const Rsrc rS0 = rsrc_of(psi0, (uint32_t)F.psi_n * ES), rS1 = rsrc_of(psi1, (uint32_t)F.psi_n * ES);
     const Rsrc rM = rsrc_of(Mk, Mk ? (uint32_t)(4 * F.nwx * ny * nz) * 4u : 0u);
     const int vl = i0c * ES;         // lane byte offset of loads (clamped lane)
-    const int vs = i0 * ES;          // lane byte offset of stores (lane_ok only)
+    const int vs = i0 * ES;          // lane byte offset of stores (owned lanes only)
     const int vm = (i0c >> 5) * 4;   // lane byte offset of its mask word
     // byte offset of row j (already clamped) of plane k in a field channel
     // (unsigned: a Float64 channel may exceed 2 GiB)
     auto rowb = [&](int j, int k) { return (int)((uint32_t)(nx * j + sxy * k) * (uint32_t)ES); };
     auto chrow = [&](int c, int rb) { return (int)((uint32_t)c * chb + (uint32_t)rb); };
     auto maskb = [&](int plane, int j, int k) { return (((plane * nz + k) * ny + j) * F.nwx) * 4; };
+    // a row store of the lane's cells (cell by cell for the partial lane)
+    auto srow = [&](Rsrc rs, int soff, const T (&o)[CPL]) {
+        if (RAG && XL.part) {
+#pragma unroll
+            for (int q = 0; q < CPL; ++q)
+                if (i0 + q < nx) {
+                    const T one[1] = {o[q]};
+                    bst<T, 1, POL>(rs, vs + q * ES, soff, one);
+                }
+        } else {
+            bst<T, CPL, POL>(rs, vs, soff, o);
+        }
+    };
 
     // per-lane x-slab data (fixed for the whole sweep)
     int sxP[CPL], lxP[CPL], sxV[CPL], lxV[CPL];
@@ -457,7 +500,7 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, in
             }
             const bool rowv = j >= 0 && j < ny;
             const bool rj = GEN && (j == ny - 1);
-            const bool ok = lane_ok && rowv;
+            const bool ok = lane_in && rowv;
             lyP[r] = lyV[r] = 0;
             syP[r] = syV[r] = 0;
             if constexpr (GEN) {
@@ -526,7 +569,7 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, in
             for (int r = 1; r <= TY; ++r) {
                 const int j = j0 + r - 1;
                 const T pml = shfl_up1<T>(p1[r][CPL - 1]);  // P1 of the previous lane's last cell
-                if (!lane_ok || j >= ny) continue;
+                if (!XL.own || j >= ny) continue;
                 const int rb = rowb(j, k);
                 const bool rj = GEN && (j == ny - 1);
                 if (cpP && !rj && !rk) {
@@ -536,8 +579,8 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, in
                 }
                 T ayV = (T)0, byV = (T)0;
                 if constexpr (GEN) {
-                    if (syP[r] && !rj) bst<T, CPL, POL>(rS1, vs, (F.mv_off(1) + nx * (lyP[r] + p * (k + nz * (syP[r] - 1)))) * ES, mPy[r]);
-                    if (szP && !rj) bst<T, CPL, POL>(rS1, vs, (F.mv_off(2) + nx * (j + ny * (lzP + p * (szP - 1)))) * ES, mPz[r]);
+                    if (syP[r] && !rj) srow(rS1, (F.mv_off(1) + nx * (lyP[r] + p * (k + nz * (syP[r] - 1)))) * ES, mPy[r]);
+                    if (szP && !rj) srow(rS1, (F.mv_off(2) + nx * (j + ny * (lzP + p * (szP - 1)))) * ES, mPz[r]);
                     if (syV[r]) { ayV = F.cah(lyV[r] + qv * (syV[r] - 1)); byV = F.cbh(lyV[r] + qv * (syV[r] - 1)); }
                 }
                 T vx1[CPL], vy1[CPL], vz1[CPL];
@@ -568,22 +611,22 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, in
                         vz1[q] = M_::upd(vzc[r][q], F.fv, F.fvf, (mvz_[r] >> q) & 1u, dp);
                     }
                 }
-                bst<T, CPL, POL>(rF1, vs, rb, p1[r]);
-                bst<T, CPL, POL>(rF1, vs, chrow(1, rb), vx1);
-                if (!GEN || j >= 1 || !F.fuse_mirror) bst<T, CPL, POL>(rF1, vs, chrow(2, rb), vy1);
-                if (!GEN || k >= 1 || !F.fuse_mirror) bst<T, CPL, POL>(rF1, vs, chrow(3, rb), vz1);
+                srow(rF1, rb, p1[r]);
+                srow(rF1, chrow(1, rb), vx1);
+                if (!GEN || j >= 1 || !F.fuse_mirror) srow(rF1, chrow(2, rb), vy1);
+                if (!GEN || k >= 1 || !F.fuse_mirror) srow(rF1, chrow(3, rb), vz1);
                 if constexpr (GEN) {
                     if (F.fuse_mirror && (j == 1 || k == 1)) {
                         T neg[CPL];
                         if (j == 1) {  // forward.jl:135
 #pragma unroll
                             for (int q = 0; q < CPL; ++q) neg[q] = -vy1[q];
-                            bst<T, CPL, POL>(rF1, vs, chrow(2, (int)((uint32_t)rb - (uint32_t)(nx * ES))), neg);
+                            srow(rF1, chrow(2, (int)((uint32_t)rb - (uint32_t)(nx * ES))), neg);
                         }
                         if (k == 1) {  // forward.jl:136
 #pragma unroll
                             for (int q = 0; q < CPL; ++q) neg[q] = -vz1[q];
-                            bst<T, CPL, POL>(rF1, vs, chrow(3, (int)((uint32_t)rb - (uint32_t)(sxy * ES))), neg);
+                            srow(rF1, chrow(3, (int)((uint32_t)rb - (uint32_t)(sxy * ES))), neg);
                         }
                     }
                 }
@@ -593,8 +636,8 @@ __device__ __forceinline__ void fused_tile(const Fused<T>& F, int j0, int k0, in
                         if (sxV[q]) psi1[F.mp_off(0) + lxV[q] + qv * (j + ny * (k + nz * (sxV[q] - 1)))] = mVx[r][q];
                 }
                 if constexpr (GEN) {
-                    if (syV[r]) bst<T, CPL, POL>(rS1, vs, (F.mp_off(1) + nx * (lyV[r] + qv * (k + nz * (syV[r] - 1)))) * ES, mVy[r]);
-                    if (szV) bst<T, CPL, POL>(rS1, vs, (F.mp_off(2) + nx * (j + ny * (lzV + qv * (szV - 1)))) * ES, mVz[r]);
+                    if (syV[r]) srow(rS1, (F.mp_off(1) + nx * (lyV[r] + qv * (k + nz * (syV[r] - 1)))) * ES, mVy[r]);
+                    if (szV) srow(rS1, (F.mp_off(2) + nx * (j + ny * (lzV + qv * (szV - 1)))) * ES, mVz[r]);
                 }
             }
         }

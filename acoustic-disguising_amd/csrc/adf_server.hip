@@ -308,8 +308,11 @@ extern "C" int adf_server_submit(adf_server* S, const adf_request* q, int64_t* i
         return fail(ADF_EINVAL, "adf_server_submit: bad dtype");
     const adf_domain& d = q->dom;
     if (d.nx < 3 || d.ny < 3 || d.nz < 3 || d.nt < 1) return fail(ADF_EINVAL, "adf_server_submit: bad domain");
-    const int64_t cpl = d.nx > 128 ? 4 : (d.nx > 64 ? 2 : 1);
-    if (d.nx > 64 * cpl || d.nx % cpl) return fail(ADF_EINVAL, "adf_server_submit: served solves need nx <= 256 and nx divisible by its cells per lane");
+    // the batched (fused) sweep's limits: one buffer descriptor per field
+    // buffer, at most 63 row segments of 62 x 4 cells
+    const int64_t wb = q->dtype == ADF_F64 ? 8 : 4;
+    if (4 * d.nx * d.ny * d.nz * wb >= (int64_t(1) << 32) || d.nx > 63 * 62 * 4)
+        return fail(ADF_EINVAL, "adf_server_submit: served solves need 4 nx ny nz sizeof(T) < 4 GiB and nx <= 15624");
     if (q->start_step < 1 || q->end_step > d.nt || q->start_step > q->end_step)
         return fail(ADF_EINVAL, "adf_server_submit: steps must satisfy 1 <= start <= end <= nt");
     if (q->mask_shape < -1 || q->mask_shape > 4) return fail(ADF_EINVAL, "adf_server_submit: bad mask_shape");

@@ -68,6 +68,8 @@ void device_pool_release(int dev) {
     pool_release_locked(dev);
 }
 
+constexpr size_t FIELD_SLACK = 256;
+
 static int dmalloc(void** p, size_t bytes, const char* what) {
     *p = nullptr;
     if (bytes == 0) return ADF_OK;
@@ -113,8 +115,12 @@ int solver_alloc(adf_solver* s, const adf_cpml* cp) {
     s->n3 = nx * ny * nz;
     s->nwx = (nx + 31) / 32;
     int rc;
+    // FIELD_SLACK bytes past each buffer: the fused sweep's last lane of a
+    // row whose length is not a multiple of its cells per lane loads (never
+    // stores) up to 3 cells past the row, unchecked at the buffer's end
+    // because the row offset is the scalar one
     for (int b = 0; b < 2; ++b)
-        if ((rc = dmalloc(&s->fieldb[b], 4 * s->n3 * s->es, "field"))) return rc;
+        if ((rc = dmalloc(&s->fieldb[b], 4 * s->n3 * s->es + FIELD_SLACK, "field"))) return rc;
     s->mv_n[0] = p * ny * nz * 2;
     s->mv_n[1] = nx * p * nz * 2;
     s->mv_n[2] = nx * ny * p * 2;
@@ -128,7 +134,7 @@ int solver_alloc(adf_solver* s, const adf_cpml* cp) {
         auto pad = [](int64_t v) { return (v + 63) / 64 * 64; };  // 256-byte aligned sub-arrays
         for (int a = 0; a < 3; ++a) { s->psi_off[a] = off; off = pad(off + s->mv_n[a]); }
         for (int a = 0; a < 3; ++a) { s->psi_off[3 + a] = off; off = pad(off + s->mp_n[a]); }
-        s->psi_n = off;
+        s->psi_n = off + 64;  // slack after the last array (as FIELD_SLACK)
     }
     for (int b = 0; b < 2; ++b) {
         if ((rc = dmalloc(&s->psib[b], s->psi_n * s->es, "cpml memory"))) return rc;
@@ -792,7 +798,8 @@ void* cur_mp(adf_solver* s, int a) { return s->mpb[s->cur][a]; }
 int solver_set_option(adf_solver* s, int option, int64_t value) {
     if (option == ADF_OPT_PATH) {
         if (value < 0 || value > 2) return fail(ADF_EINVAL, "ADF_OPT_PATH must be 0, 1 or 2");
-        if (value == 2 && !fused_supported(s)) return fail(ADF_EINVAL, "fused path needs 2 <= nx <= 256");
+        if (value == 2 && !fused_supported(s))
+            return fail(ADF_EINVAL, "fused path needs 4 nx ny nz sizeof(T) < 4 GiB and at most 63 row segments");
         s->path = (int)value;
         touch(s);
     } else if (option == ADF_OPT_TILE_Y) {
@@ -858,6 +865,27 @@ static Solve<T> make_solve(adf_solver* s) {
     return S;
 }
 
+// Row segments of the fused kernel (xlane): a row of up to 64 cpl cells is
+// one wave's; a wider one takes the fewest waves of at most 62 cpl owned
+// cells each, with the cells shared out evenly (a multiple of cpl each).
+static int64_t fused_nseg(int64_t nx, int cpl) { return nx <= 64 * cpl ? 1 : (nx + 62 * cpl - 1) / (62 * cpl); }
+static int fused_xseg(int64_t nx, int cpl) {
+    const int64_t n = fused_nseg(nx, cpl);
+    if (n == 1) return 64 * cpl;
+    return (int)(((nx + n - 1) / n + cpl - 1) / cpl * cpl);
+}
+
+// Cells per lane of the fused kernel.  It addresses a field buffer (4
+// channels) through one buffer descriptor, so it needs 4 n3 sizeof(T) <
+// 4 GiB, and a tile entry holds at most 63 row segments.
+static int fused_cpl(const adf_solver* s) {
+    const int64_t nx = s->dom.nx;
+    const int cpl = nx > 128 ? 4 : (nx > 64 ? 2 : 1);
+    if (fused_nseg(nx, cpl) > 63) return 0;
+    if (4 * s->n3 * (int64_t)s->es >= (int64_t(1) << 32)) return 0;
+    return cpl;
+}
+
 template <typename T>
 static Fused<T> make_fused(adf_solver* s, int ty) {
     Fused<T> F;
@@ -878,6 +906,7 @@ static Fused<T> make_fused(adf_solver* s, int ty) {
         F.off_mp[a] = (int)s->psi_off[3 + a];
     }
     F.ty = ty;
+    F.xseg = fused_xseg(s->dom.nx, std::max(fused_cpl(s), 1));
     F.dx = s->dom.dx; F.dy = s->dom.dy; F.dz = s->dom.dz;
     F.fq = s->dom.fact_m0; F.fv = s->dom.fact_m1;
     F.rdx = (float)(1.0 / s->dom.dx); F.rdy = (float)(1.0 / s->dom.dy); F.rdz = (float)(1.0 / s->dom.dz);
@@ -894,17 +923,6 @@ static Txs<T> make_txs(adf_solver* s) {
     return X;
 }
 
-// Cells per lane of the fused kernel: one wave64 spans the whole x-row.
-// The fused kernel addresses a field buffer (4 channels) through one
-// buffer descriptor, so it needs 4 n3 sizeof(T) < 4 GiB.
-static int fused_cpl(const adf_solver* s) {
-    const int64_t nx = s->dom.nx;
-    const int cpl = nx > 128 ? 4 : (nx > 64 ? 2 : 1);
-    if (nx > 64 * cpl || nx % cpl != 0) return 0;
-    if (4 * s->n3 * (int64_t)s->es >= (int64_t(1) << 32)) return 0;
-    return cpl;
-}
-
 bool fused_supported(const adf_solver* s) { return fused_cpl(s) > 0; }
 
 // Tile shape: TY rows per interior tile, KB planes per interior tile (halved
@@ -917,7 +935,7 @@ void pick_tiles(const adf_solver* s, int* ty, int* kb, int* kbg) {
     if (!k) {
         const int64_t ny = s->dom.ny, nz = s->dom.nz;
         const int64_t want = 16LL * s->ctx->num_cu;
-        const int64_t tiles_y = (ny + t - 1) / t;
+        const int64_t tiles_y = (ny + t - 1) / t * fused_nseg(s->dom.nx, std::max(fused_cpl(s), 1));
         k = 64;
         while (k > 8 && tiles_y * ((nz + k - 1) / k) < want) k /= 2;
     }
@@ -1063,8 +1081,9 @@ static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, int W, uin
     const int ny = (int)s->dom.ny, nz = (int)s->dom.nz, p = (int)s->p, qv = p + 1;
     const bool cpP = s->p > 0 && (s->has_mask || s->p > 1), cpV = qv > 1;
     const int tiles_y = (ny + ty - 1) / ty;
-    kb = std::min(kb, 65535);
-    kbg = std::min(kbg, 65535);
+    const int nseg = (int)fused_nseg(s->dom.nx, std::max(fused_cpl(s), 1));
+    kb = std::min(kb, 1023);
+    kbg = std::min(kbg, 1023);
     // regular rows / planes: owned ones >= 2 (no mirror), the halo >= 1,
     // and nothing at or past n-1 (rigid plane, edge)
     auto rgen = [&](int t) { const int j0 = t * ty; return j0 < 2 || j0 + ty > ny - 1; };
@@ -1086,14 +1105,14 @@ static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, int W, uin
     };
     // a tile needs the mask if a row of it or of its halo (rows j0-1..j0+ty,
     // planes k0-1..k1) is not fully open
-    // entry of the tile of rows [j0, j0+h), planes [k0, k1)
-    auto tile = [&](int j0, int h, int k0, int k1, uint32_t kind) {
+    // entry of the tile of rows [j0, j0+h), planes [k0, k1), row segment seg
+    auto tile = [&](int j0, int h, int k0, int k1, uint32_t kind, int seg) {
         bool all_open = true;
         for (int k = std::max(k0 - 1, 0); k <= std::min(k1, nz - 1) && all_open; ++k)
             for (int j = std::max(j0 - 1, 0); j <= std::min(j0 + h, ny - 1); ++j)
                 if (!open[(size_t)k * ny + j]) { all_open = false; break; }
         return make_uint2((all_open ? 0u : 0x80000000u) | (kind << 28) | (m << 16) | (uint32_t)j0,
-                          ((uint32_t)k0 << 16) | (uint32_t)(k1 - k0));
+                          ((uint32_t)k0 << 16) | ((uint32_t)seg << 10) | (uint32_t)(k1 - k0));
     };
     // z ranges: general [0, 2), regular [2, nz-1) split at the z slabs
     // (interior planes [zi0, zi1): halo and owned planes clear of both
@@ -1135,18 +1154,20 @@ static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, int W, uin
                 kind = ys ? (zl ? TILE_SYZ : TILE_SY) : (zl ? TILE_SZ : TILE_INT);
             }
             if (kind == TILE_INT) {
-                lists[LIST_INT].emplace_back(z.first, j0, tile(j0, ty, z.first, z.second, kind));
+                for (int g = 0; g < nseg; ++g)
+                    lists[LIST_INT].emplace_back(z.first, j0, tile(j0, ty, z.first, z.second, kind, g));
                 continue;
             }
             for (int j = j0; j < j1; j += tys)
-                lists[LIST_SIDE].emplace_back(z.first, j, tile(j, tys, z.first, z.second, kind));
+                for (int g = 0; g < nseg; ++g)
+                    lists[LIST_SIDE].emplace_back(z.first, j, tile(j, tys, z.first, z.second, kind, g));
         }
     }
     auto order = [](const std::tuple<int, int, uint2>& a, const std::tuple<int, int, uint2>& b) {
         return std::get<0>(a) != std::get<0>(b) ? std::get<0>(a) < std::get<0>(b) : std::get<1>(a) < std::get<1>(b);
     };
     for (int c : {LIST_INT, LIST_SIDE}) {
-        std::sort(lists[c].begin(), lists[c].end(), order);
+        std::stable_sort(lists[c].begin(), lists[c].end(), order);  // segments of a row stay adjacent
         out[c].clear();
         for (auto& e : lists[c]) out[c].push_back(std::get<2>(e));
     }
@@ -1444,7 +1465,9 @@ int batch_add(adf_batch* b, adf_solver* s) {
     if (s->ctx != b->ctx) return fail(ADF_EINVAL, "adf_batch_add: solver uses another context");
     if (!b->members.empty() && s->dtype != b->dtype) return fail(ADF_EINVAL, "adf_batch_add: batch members share one dtype");
     if (s->gf_kind) return fail(ADF_EINVAL, "adf_batch_add: solvers with a GF map run alone (adf_solver_run)");
-    if (!fused_supported(s)) return fail(ADF_EINVAL, "adf_batch_add: batched solves need nx <= 256 with nx % (cells per lane) == 0");
+    if (!fused_supported(s))
+        return fail(ADF_EINVAL, "adf_batch_add: batched solves need the fused sweep (4 nx ny nz sizeof(T) < 4 GiB, "
+                                "at most 63 row segments)");
     if ((int64_t)b->members.size() >= 4096) return fail(ADF_EINVAL, "adf_batch_add: at most 4096 members");
     if (s->ntx >= (1 << 19)) return fail(ADF_EINVAL, "adf_batch_add: at most 524287 transceivers per member");
     if (b->members.empty()) b->dtype = s->dtype;
@@ -1472,7 +1495,7 @@ static int batch_build(adf_batch* b) {
     std::vector<Fused<T>> fs(2 * n);
     std::vector<TxDev<T>> td(n);
     std::vector<int> it0(n), cur(n);
-    std::map<std::tuple<int, int, int, int, int>, std::array<std::vector<uint2>, TILE_LISTS>> glists;
+    std::map<std::tuple<int, int, int, int, int, int>, std::array<std::vector<uint2>, TILE_LISTS>> glists;
     // total work decides the z-block: enough tiles for every CU
     int64_t cells = 0;
     for (adf_solver* s : b->members) cells += s->n3;
@@ -1487,7 +1510,8 @@ static int batch_build(adf_batch* b) {
             while (kb > 2) {
                 int64_t tiles = 0;
                 for (adf_solver* o : b->members)
-                    tiles += ((o->dom.ny + ty - 1) / ty) * ((o->dom.nz + kb - 1) / kb);
+                    tiles += ((o->dom.ny + ty - 1) / ty) * ((o->dom.nz + kb - 1) / kb) *
+                             fused_nseg(o->dom.nx, std::max(fused_cpl(o), 1));
                 if (tiles >= 8LL * b->ctx->num_cu) break;
                 kb /= 2;
             }
@@ -1513,8 +1537,10 @@ static int batch_build(adf_batch* b) {
         if ((rc0 = open_rows(s, open))) return rc0;
         std::vector<uint2> lists[TILE_LISTS];
         split_tiles(s, ty, kb, kbg, 1, (uint32_t)m, open, lists);
-        const std::tuple<int, int, int, int, int> key(fused_cpl(s), ty, s->waves ? s->waves : 1, s->pipeline,
-                                                      s->launch == ADF_SWEEP_ONE_LAUNCH ? 1 : 0);
+        const int cpl = fused_cpl(s);
+        const std::tuple<int, int, int, int, int, int> key(cpl, ty, s->waves ? s->waves : 1, s->pipeline,
+                                                           s->launch == ADF_SWEEP_ONE_LAUNCH ? 1 : 0,
+                                                           s->dom.nx % cpl != 0 ? 1 : 0);
         auto& G = glists[key];
         for (int c : {LIST_INT, LIST_SIDE}) G[c].insert(G[c].end(), lists[c].begin(), lists[c].end());
     }
@@ -1534,6 +1560,7 @@ static int batch_build(adf_batch* b) {
         g.waves = b->persistent ? 1 : std::get<2>(kv.first);
         g.pf = std::get<3>(kv.first);
         g.merged = std::get<4>(kv.first);
+        g.rag = std::get<5>(kv.first);
         if (g.merged) merge_lists(kv.second[LIST_SIDE], kv.second[LIST_INT], g.waves, kv.second[LIST_ALL]);
         for (int h = 0; h < TILE_LISTS; ++h) {
             const std::vector<uint2>& L = kv.second[h];

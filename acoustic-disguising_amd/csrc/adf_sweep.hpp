@@ -64,17 +64,15 @@ struct IPlane {
     uint32_t mp_[TY + 1], mvx_[TY + 1], mvy_[TY + 1], mvz_[TY + 1];
 };
 
-template <typename T, bool EXACT, int CPL, int TY>
-__device__ __forceinline__ void sweep_int_tile(const Fused<T>& F, int j0, int k0, int k1, bool masked) {
+template <typename T, bool EXACT, int CPL, int TY, bool RAG = false>
+__device__ __forceinline__ void sweep_int_tile(const Fused<T>& F, int j0, int k0, int k1, bool masked, int seg) {
     typedef FA<T, EXACT> M_;
     typedef typename M_::A A;
     constexpr int ES = (int)sizeof(T);
     const int nx = F.nx, ny = F.ny, nz = F.nz, p = F.p, qv = p + 1;
     const int sxy = nx * ny;
-    const int lane = threadIdx.x & 63;
-    const int i0 = lane * CPL;
-    const bool lane_ok = i0 < nx;
-    const int i0c = lane_ok ? i0 : nx - CPL;
+    const XLane XL = xlane<CPL>(nx, F.xseg, seg);
+    const int i0 = XL.i0, i0c = XL.i0c;
     const bool cpP = F.cpml_p != 0, cpV = qv > 1;
 
     const uint32_t chb = (uint32_t)F.n3 * (uint32_t)ES;
@@ -169,12 +167,22 @@ __device__ __forceinline__ void sweep_int_tile(const Fused<T>& F, int j0, int k0
                     vz1[q] = M_::upd(vzc[r][q], F.fv, F.fvf, (mvz >> q) & 1u,
                                      M_::diff(p1[r][q], p1prev[r][q], F.dz, F.rdz));
                 }
-                if (!lane_ok) continue;
+                if (!XL.own) continue;
                 const int vs = i0 * ES;
-                bst<T, CPL>(rF1, vs, rb, p1[r]);
-                bst<T, CPL>(rF1, vs, chrow(1, rb), vx1);
-                bst<T, CPL>(rF1, vs, chrow(2, rb), vy1);
-                bst<T, CPL>(rF1, vs, chrow(3, rb), vz1);
+                if (RAG && XL.part) {
+#pragma unroll
+                    for (int q = 0; q < CPL; ++q)
+                        if (i0 + q < nx) {
+                            const T o[4][1] = {{p1[r][q]}, {vx1[q]}, {vy1[q]}, {vz1[q]}};
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) bst<T, 1>(rF1, vs + q * ES, chrow(c, rb), o[c]);
+                        }
+                } else {
+                    bst<T, CPL>(rF1, vs, rb, p1[r]);
+                    bst<T, CPL>(rF1, vs, chrow(1, rb), vx1);
+                    bst<T, CPL>(rF1, vs, chrow(2, rb), vy1);
+                    bst<T, CPL>(rF1, vs, chrow(3, rb), vz1);
+                }
                 // CPML memory of the slab cells
                 if (anyP) {
                     const int so = p * (j + ny * k) * ES;
@@ -257,18 +265,24 @@ struct SPlane : IPlane<T, CPL, TY> {
 // 320-378,380-501).  The x slab is handled the same way per lane.
 // NOMASK: the instantiation for tiles that meet no masked cell (no mask
 // loads or registers).
-template <typename T, bool EXACT, int CPL, int TY, bool SY, bool SZ, int POL = 0, bool NOMASK = false>
-__device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0, int k1, bool masked) {
+template <typename T, bool EXACT, int CPL, int TY, bool SY, bool SZ, int POL = 0, bool NOMASK = false,
+          bool RAG = false>
+__device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0, int k1, bool masked, int seg) {
     typedef FA<T, EXACT> M_;
     typedef typename M_::A A;
     typedef SPlane<T, CPL, TY, SY, SZ> Plane;
     constexpr int ES = (int)sizeof(T);
     const int nx = F.nx, ny = F.ny, nz = F.nz, p = F.p, qv = p + 1;
     const int sxy = nx * ny;
-    const int lane = threadIdx.x & 63;
-    const int i0 = lane * CPL;
-    const bool lane_ok = i0 < nx;
-    const int i0c = lane_ok ? i0 : nx - CPL;
+    const XLane XL = xlane<CPL>(nx, F.xseg, seg);
+    const int i0 = XL.i0, i0c = XL.i0c;
+    // lanes storing one vector per row; with RAG the partial lane stores
+    // cell by cell at offsets vq (out of range for every other lane)
+    const bool lane_ok = XL.own && !(RAG && XL.part);
+    int vq[RAG ? CPL - 1 : 1];
+#pragma unroll
+    for (int q = 0; q < (RAG ? CPL - 1 : 1); ++q)
+        vq[q] = (RAG && XL.part && i0 + q < nx) ? (i0 + q) * ES : (int)OOR;
     const bool cpP = F.cpml_p != 0, cpV = qv > 1;
 
     const uint32_t chb = (uint32_t)F.n3 * (uint32_t)ES;
@@ -303,7 +317,7 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
     // CPML memory stores of the x-slab cells: one vector store for lanes
     // whose cells all lie in the slab, one element store per cell for lanes
     // partly in it; the other lanes' offsets are out of range
-    const bool stP = lane_ok && cpP, stV = lane_ok && cpV;
+    const bool stP = XL.own && cpP, stV = XL.own && cpV;
     const int sPv = (stP && XP.in == FULL) ? vP : (int)OOR;
     const int sVv = (stV && XV.in == FULL) ? vV : (int)OOR;
     int sPq[CPL], sVq[CPL];
@@ -467,6 +481,15 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
             bst<T, CPL, POL>(rF1, vsc[1], rb, vx1);
             bst<T, CPL, POL>(rF1, vsc[2], rb, vy1);
             bst<T, CPL, POL>(rF1, vsc[3], rb, vz1);
+            if constexpr (RAG) {
+#pragma unroll
+                for (int q = 0; q < CPL - 1; ++q) {
+                    const T o[4][1] = {{p1[r][q]}, {vx1[q]}, {vy1[q]}, {vz1[q]}};
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        bst<T, 1, POL>(rF1, vq[q], (int)((uint32_t)rb + (uint32_t)c * chb), o[c]);
+                }
+            }
             const int soP = p * (j + ny * k) * ES, soV = qv * (j + ny * k) * ES;
             bst<T, CPL, POL>(rS1, sPv, soP, L.mPx[r]);
             bst<T, CPL, POL>(rS1, sVv, soV, L.mVx[r]);
@@ -476,13 +499,24 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
                 bst<T, 1, POL>(rS1, sPq[q], soP, a1);
                 bst<T, 1, POL>(rS1, sVq[q], soV, b1);
             }
+            // y/z slab memory (row vectors of the slab arrays)
+            auto sslab = [&](Rsrc rs, int soff, const T (&o)[CPL]) {
+                bst<T, CPL, POL>(rs, vs, soff, o);
+                if constexpr (RAG) {
+#pragma unroll
+                    for (int q = 0; q < CPL - 1; ++q) {
+                        const T one[1] = {o[q]};
+                        bst<T, 1, POL>(rs, vq[q], soff, one);
+                    }
+                }
+            };
             if constexpr (SY) {
-                bst<T, CPL, POL>(rsrc_of(F.psi1, yinP[r] ? psib : 0u), vs, yoP[r] + k * ypP, L.mPy[r]);
-                bst<T, CPL, POL>(rsrc_of(F.psi1, yinV[r] ? psib : 0u), vs, yoV[r] + k * ypV, L.mVy[r]);
+                sslab(rsrc_of(F.psi1, yinP[r] ? psib : 0u), yoP[r] + k * ypP, L.mPy[r]);
+                sslab(rsrc_of(F.psi1, yinV[r] ? psib : 0u), yoV[r] + k * ypV, L.mVy[r]);
             }
             if constexpr (SZ) {
-                bst<T, CPL, POL>(rsrc_of(F.psi1, L.zinP ? psib : 0u), vs, L.zoP + r * nx * ES, L.mPz[r]);
-                bst<T, CPL, POL>(rsrc_of(F.psi1, L.zinV ? psib : 0u), vs, L.zoV + r * nx * ES, L.mVz[r]);
+                sslab(rsrc_of(F.psi1, L.zinP ? psib : 0u), L.zoP + r * nx * ES, L.mPz[r]);
+                sslab(rsrc_of(F.psi1, L.zinV ? psib : 0u), L.zoV + r * nx * ES, L.mVz[r]);
             }
         }
         carry(L, p1);
@@ -528,40 +562,42 @@ enum { LIST_INT = 0, LIST_SIDE = 1, LIST_ALL = 2, TILE_LISTS = 3 };
 // One tile of a solve.  PF: the pipelined routine for every regular tile;
 // otherwise the plane-at-a-time routines (fused_tile for all side tiles).
 // General tiles always take fused_tile.
-template <typename T, bool EXACT, int CPL, bool PF, int POL = 0>
+// RAG: the solve's rows are not a multiple of CPL long (fused_tile).
+template <typename T, bool EXACT, int CPL, bool PF, int POL = 0, bool RAG = false>
 __device__ __forceinline__ void sweep_tile(const Fused<T>& F, uint2 e) {
-    const int j0 = (int)(e.x & 0xffffu), k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0xffffu);
+    const int j0 = (int)(e.x & 0xffffu), k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0x3ffu);
+    const int seg = (int)((e.y >> 10) & 0x3fu);
     const bool masked = (e.x >> 31) != 0;
     const int kind = (int)((e.x >> 28) & 7u);
     constexpr int TI = sweep_ty_int((int)sizeof(T), CPL), TS = sweep_ty_side(CPL);
     if (kind == TILE_INT) {
         if constexpr (PF) {
-            if (masked) sweep_tile_pf<T, EXACT, CPL, TI, false, false, POL>(F, j0, k0, k1, true);
-            else sweep_tile_pf<T, EXACT, CPL, TI, false, false, POL, true>(F, j0, k0, k1, false);
+            if (masked) sweep_tile_pf<T, EXACT, CPL, TI, false, false, POL, false, RAG>(F, j0, k0, k1, true, seg);
+            else sweep_tile_pf<T, EXACT, CPL, TI, false, false, POL, true, RAG>(F, j0, k0, k1, false, seg);
         } else {
-            sweep_int_tile<T, EXACT, CPL, TI>(F, j0, k0, k1, masked);
+            sweep_int_tile<T, EXACT, CPL, TI, RAG>(F, j0, k0, k1, masked, seg);
         }
     } else if (PF && kind == TILE_SY) {
-        sweep_tile_pf<T, EXACT, CPL, TS, true, false, POL>(F, j0, k0, k1, masked);
+        sweep_tile_pf<T, EXACT, CPL, TS, true, false, POL, false, RAG>(F, j0, k0, k1, masked, seg);
     } else if (PF && kind == TILE_SZ) {
-        sweep_tile_pf<T, EXACT, CPL, TS, false, true, POL>(F, j0, k0, k1, masked);
+        sweep_tile_pf<T, EXACT, CPL, TS, false, true, POL, false, RAG>(F, j0, k0, k1, masked, seg);
     } else if (PF && kind == TILE_SYZ) {
-        sweep_tile_pf<T, EXACT, CPL, TS, true, true, POL>(F, j0, k0, k1, masked);
+        sweep_tile_pf<T, EXACT, CPL, TS, true, true, POL, false, RAG>(F, j0, k0, k1, masked, seg);
     } else {
-        fused_tile<T, EXACT, CPL, TS, true, POL>(F, j0, k0, k1, masked);
+        fused_tile<T, EXACT, CPL, TS, true, POL, RAG>(F, j0, k0, k1, masked, seg);
     }
 }
 
 // One solve's tiles of one list (entries as xcd_tile describes); W waves
 // per workgroup take W consecutive tiles.  POL: cache policy of the
 // stores (0 or ST_NT).
-template <typename T, bool EXACT, int CPL, bool PF, int POL>
+template <typename T, bool EXACT, int CPL, bool PF, int POL, bool RAG>
 __global__ __launch_bounds__(256) void k_sweep(Fused<T> F, const uint2* __restrict__ tiles, int ntiles) {
     const int W = (int)(blockDim.x >> 6);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x) * W + w;
     if (t >= ntiles) return;
-    sweep_tile<T, EXACT, CPL, PF, POL>(F, tiles[t]);
+    sweep_tile<T, EXACT, CPL, PF, POL, RAG>(F, tiles[t]);
 }
 
 }  // namespace adf

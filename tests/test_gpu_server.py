@@ -102,8 +102,9 @@ def test_bad_requests_fail_cleanly():
         with pytest.raises(L.AdfError, match="steps"):
             sv.submit(d, code, A.make_cpml(d, cpml.n, cpml.rcoef, cpml.fc, np.float32),
                       A.transceivers_to_grid(d, txs, np.float32), start_step=5, end_step=d.nt + 1)
-        wide = A.make_domain(1e-3, 0.1, 0.02, 0.02, 260, 8, 8, 343.0, 1.21, 0.5)
-        with pytest.raises(L.AdfError, match="nx <= 256"):
+        # rows of more than 63 segments of 62 x 4 cells
+        wide = A.make_domain(1e-3, 10.0, 0.02, 0.02, 15625, 4, 4, 343.0, 1.21, 0.5)
+        with pytest.raises(L.AdfError, match="nx <= 15624"):
             sv.submit(wide, L.ADF_F32, None, None)
         rid = submit(sv, requests()[0], want=0x1, keep_field=False)
         assert sv.wait(rid, 60.0) == "done"
