@@ -20,6 +20,8 @@ Modes for profiling (see profiles/):
   --profile-gf        run the in-loop KH gather leg only
   --profile-hologram  run the 192^3 hologram synthesis leg only
   --profile-kh        run the post-hoc KH leg only (p_v, 2000 -> 500 points, 2000 steps)
+  --profile-grid N    run the production-grid leg at N^3 only (301, 501 or 545;
+                      --profile-steps K steps, default 50)
   --calibrate-copy    HBM ceilings of the library's streaming probe kernels
                       (copy/read/write; --probe-kind K for one kind, the
                       run the PMC scale factors are read from)
@@ -191,6 +193,47 @@ def gpu_leg(args):
                      "p_rec_peak": peak}
         del s
     return d, p, out
+
+
+PAPER_GRIDS = (301, 501, 545)
+
+
+def paper_grid_leg(n, steps=50, warmup=10):
+    """The reference's production grid sizes (configs/paper.toml:21,71,165:
+    [grid] n = 501 hologram box, [greens.grid] n = 301, [scattering.grid]
+    n = 545), rows wider than one wave and not a multiple of its cells per
+    lane: n^3 cells over a 2.2 m box with rigid walls (paper.toml's pml
+    n = 0), a rigid sphere of radius 0.1 m, a Ricker point source at the
+    centre (fc 10 kHz) and p+vn recorded on the 400-point inner (0.2 m) and
+    outer (0.3 m) Fibonacci surfaces.  FP32, fused sweep, autotuned tiling."""
+    import adfdtd as A
+    from adfdtd import _lib as L
+
+    nt = warmup + steps + 1
+    d0 = A.make_domain(1.0, 1.1, 1.1, 1.1, n, n, n, C_AIR, RHO_AIR, 0.5)
+    d = A.make_domain((nt - 0.5) * d0.dt, 1.1, 1.1, 1.1, n, n, n, C_AIR, RHO_AIR, 0.5)
+    tf = A.ricker_wavelet(d, 10e3, True)
+    z = np.zeros(d.nt)
+    txs = [A.make_transceiver(d, [0.0, 0.0, 0.0], [1.0, 0.0, 0.0, 0.0], tf)]
+    for r in (0.2, 0.3):
+        pts, nrm = A.fibonacci_sphere(400, r)
+        txs += [A.make_transceiver(d, pts[i], np.r_[0.0, nrm[i]], z) for i in range(400)]
+    s = A.Solver(d, A.dtype_code(np.float32, "fast"), None)
+    s.build_mask("sphere", [0.1])
+    shape = A.tune_solver(s)
+    s.set_transceivers(A.transceivers_to_grid(d, txs, np.float32))
+    s.run(1, warmup)
+    s.set_option(L.ADF_OPT_TIME_SWEEP, sweep_sample_period(steps))
+    t0 = time.perf_counter()
+    s.run(warmup + 1, warmup + steps)
+    wall = time.perf_counter() - t0
+    sweep_ms, launches = s.sweep_stats()
+    b = algorithmic_bytes_per_step(n, 0, 4)
+    del s
+    return {"grid": [n, n, n], "steps": steps, "ms_per_step": round(wall / steps * 1e3, 4),
+            "glups": round(n ** 3 * steps / wall / 1e9, 2), "sweep_ms": round(sweep_ms, 4),
+            "sweep_frac": round(b / (sweep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes_per_launch": b, "launches_timed": launches, "tiling": shape}
 
 
 def host_cpu():
@@ -609,6 +652,8 @@ def parse_args(argv=None):
     ap.add_argument("--profile-gf", action="store_true")
     ap.add_argument("--profile-hologram", action="store_true")
     ap.add_argument("--profile-kh", action="store_true")
+    ap.add_argument("--profile-grid", type=int, default=0)
+    ap.add_argument("--skip-paper", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--probe-kind", type=int, default=-1, help="with --calibrate-copy: one probe kind only")
     ap.add_argument("--probe-bytes", type=int, default=1 << 30)
@@ -642,12 +687,15 @@ def main():
     if args.profile_kh:
         print(json.dumps(kh_leg()))
         return
+    if args.profile_grid:
+        print(json.dumps(paper_grid_leg(args.profile_grid, steps=args.profile_steps or 50)))
+        return
     if args.profile_batch:
         print(json.dumps(batched_leg(args.batch_solves, [128], args.profile_batch)))
         return
     if args.profile_steps:
         args.steps, args.warmup, args.fp64_steps = args.profile_steps, 2, 0
-        args.skip_cpu = args.skip_batch = args.skip_served = True
+        args.skip_cpu = args.skip_batch = args.skip_served = args.skip_paper = True
 
     d, p, g = gpu_leg(args)
     log("256^3 plain-path legs done")
@@ -725,6 +773,12 @@ def main():
         res["kh_post_hoc"] = {"workload": "analytic KH p_v kernel, 2000-point KH sphere -> 500 far-field points, "
                                           "2000 steps, FP32 kernel samples and inputs, double sums", **kh_leg()}
         log("post-hoc KH leg done")
+    if not args.skip_paper:
+        res["paper_grids"] = {"workload": "the reference's production grid sizes (paper.toml 301/501/545^3): 2.2 m "
+                                          "box, rigid walls, sphere r=0.1 m, centre Ricker source (10 kHz), p+vn "
+                                          "on 400+400 surface points, FP32, 50 steps",
+                              "runs": [paper_grid_leg(n) for n in PAPER_GRIDS]}
+        log("production-grid legs done")
     if not args.skip_served:
         res["served"] = {"workload": "seeded Poisson arrivals (20 req/s) of FP32 impulsive-GF requests: grid "
                                      "96/128/192/256^3, 800-3000 steps, sphere/cube/ellipsoid r=10-30 % of the "
