@@ -180,8 +180,10 @@ int adf_solver_run(adf_solver* s, int64_t start_step, int64_t end_step);
 int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* out);
 
 /* Kernel path and tiling (tuning knobs; results do not depend on them):
- *   ADF_OPT_PATH     0 auto (fused when nx <= 256), 1 two-pass P/V kernels,
- *                    2 fused z-marching kernel
+ *   ADF_OPT_PATH     0 auto (fused when a field buffer is under 4 GiB and a
+ *                    row needs at most 63 segments of 248 cells, i.e.
+ *                    nx <= 15624), 1 two-pass P/V kernels, 2 fused
+ *                    z-marching kernel
  *   ADF_OPT_TILE_Y   rows per wave of the fused kernel: 0 auto, 2 (the only
  *                    shape built)
  *   ADF_OPT_BLOCK_Z  planes per workgroup of the fused kernel: 0 auto, >= 1 */
@@ -262,8 +264,8 @@ int adf_solver_next_step(adf_solver* s, int64_t* it);
  * (interior tiles, side tiles) per row-width class, the transceiver levels
  * of all members as one launch per level, then the velocity fixups and
  * mirrors; it is captured as a HIP graph and replayed.
- * Members: same context and dtype, nx <= 256 and nx divisible by its cells
- * per lane (1 up to nx 64, 2 up to 128, 4 up to 256), no GF map. */
+ * Members: same context and dtype, a fused sweep (4 nx ny nz sizeof(T) <
+ * 4 GiB, nx <= 15624), no GF map. */
 int adf_batch_create(adf_ctx* ctx, adf_batch** out);
 /* Members are released (not destroyed). */
 int adf_batch_destroy(adf_batch* b);
