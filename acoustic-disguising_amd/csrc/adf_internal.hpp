@@ -195,6 +195,8 @@ struct adf_batch {
 namespace adf {
 // Frees the device allocations pooled for reuse on device dev.
 void device_pool_release(int dev);
+// Frees pooled allocations of device dev until at most keep bytes stay pooled.
+void device_pool_trim(int dev, size_t keep);
 // A change to a member's schedule, mask or tiling invalidates its batch's
 // device tables.
 inline void touch(adf_solver* s) {

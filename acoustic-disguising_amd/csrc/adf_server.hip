@@ -218,28 +218,35 @@ void loop(adf_server* S) {
         const std::string msg = rc ? std::string(adf_last_error()) : std::string();
         std::vector<int> rrc(done.size());
         for (size_t i = 0; i < done.size(); ++i) rrc[i] = rc ? rc : retire(S, done[i]);
-        std::lock_guard<std::mutex> lk(S->mu);
-        S->busy_s += t1 - t0;
-        S->lups += lups;
-        S->live = keep;
-        const double t = now_s();
-        for (size_t i = 0; i < done.size(); ++i) {
-            Request* r = done[i];
-            r->t_done = t;
-            S->in_use -= r->bytes;
-            if (rrc[i]) {
-                if (r->s) { adf_solver_destroy(r->s); r->s = nullptr; }
-                if (r->err.empty()) r->err = msg;
-                r->status = ADF_REQ_FAILED;
-                S->failed++;
-            } else {
-                r->status = ADF_REQ_DONE;
-                S->completed++;
-                S->lat_ms.push_back((r->t_done - r->t_submit) * 1e3);
-                S->t_last = t;
+        uint64_t headroom = 0;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->busy_s += t1 - t0;
+            S->lups += lups;
+            S->live = keep;
+            const double t = now_s();
+            for (size_t i = 0; i < done.size(); ++i) {
+                Request* r = done[i];
+                r->t_done = t;
+                S->in_use -= r->bytes;
+                if (rrc[i]) {
+                    if (r->s) { adf_solver_destroy(r->s); r->s = nullptr; }
+                    if (r->err.empty()) r->err = msg;
+                    r->status = ADF_REQ_FAILED;
+                    S->failed++;
+                } else {
+                    r->status = ADF_REQ_DONE;
+                    S->completed++;
+                    S->lat_ms.push_back((r->t_done - r->t_submit) * 1e3);
+                    S->t_last = t;
+                }
             }
+            S->cv_done.notify_all();
+            headroom = S->budget > S->in_use ? S->budget - S->in_use : 0;
         }
-        S->cv_done.notify_all();
+        // the blocks the retired requests freed stay pooled for the next
+        // admissions only within the budget's headroom
+        if (!done.empty()) device_pool_trim(S->ctx->device, (size_t)headroom);
     }
 }
 
@@ -259,9 +266,11 @@ extern "C" int adf_server_create(int device, uint64_t hbm_budget_bytes, adf_serv
     adf_server* S = new adf_server();
     S->ctx = ctx;
     S->budget = hbm_budget_bytes;
+    // blocks pooled by earlier solves are released: the budget bounds live
+    // requests plus what the pool keeps for them (see the loop)
+    device_pool_release(device);
     if (!S->budget) {
         size_t fr = 0, tot = 0;
-        device_pool_release(device);  // pooled blocks count as free memory here
         if (hipMemGetInfo(&fr, &tot) == hipSuccess) S->budget = (uint64_t)(0.85 * (double)fr);
     }
     S->th = std::thread(loop, S);

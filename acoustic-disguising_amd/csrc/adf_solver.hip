@@ -42,30 +42,41 @@ namespace {
 std::mutex pool_mu;
 std::unordered_map<void*, std::pair<int, size_t>> pool_live;  // block -> (device, bytes)
 std::map<std::pair<int, size_t>, std::vector<std::pair<void*, uint64_t>>> pool_free;  // (block, free count)
-size_t pool_bytes = 0;
-uint64_t pool_frees = 0;   // blocks pooled so far
-uint64_t pool_synced = 0;  // pool_frees at the start of the last device synchronization
+size_t pool_bytes = 0;                        // all devices (POOL_MAX applies to the sum)
+std::unordered_map<int, size_t> pool_dev_bytes;  // per device
+uint64_t pool_frees = 0;                      // blocks pooled so far
+// per device: pool_frees at the start of its last synchronization
+std::unordered_map<int, uint64_t> pool_synced;
 constexpr size_t POOL_MAX = (size_t)32 << 30;
 
-void pool_release_locked(int dev) {
-    for (auto it = pool_free.begin(); it != pool_free.end();) {
-        if (it->first.first != dev) {
-            ++it;
-            continue;
-        }
-        for (auto& b : it->second) {
-            (void)hipFree(b.first);
+// Frees pooled blocks of device dev until at most keep bytes of it stay
+// pooled (largest blocks first).
+void pool_trim_locked(int dev, size_t keep) {
+    for (auto it = pool_free.end(); it != pool_free.begin() && pool_dev_bytes[dev] > keep;) {
+        --it;
+        if (it->first.first != dev) continue;
+        auto& v = it->second;
+        while (!v.empty() && pool_dev_bytes[dev] > keep) {
+            (void)hipFree(v.back().first);
+            v.pop_back();
             pool_bytes -= it->first.second;
+            pool_dev_bytes[dev] -= it->first.second;
         }
-        it = pool_free.erase(it);
+        if (v.empty()) it = pool_free.erase(it);
     }
 }
 }  // namespace
 
-// Frees the pooled blocks of device dev (the current device).
+// Frees the pooled blocks of device dev.
 void device_pool_release(int dev) {
     std::lock_guard<std::mutex> lk(pool_mu);
-    pool_release_locked(dev);
+    pool_trim_locked(dev, 0);
+}
+
+// Frees pooled blocks of device dev until at most keep bytes stay pooled.
+void device_pool_trim(int dev, size_t keep) {
+    std::lock_guard<std::mutex> lk(pool_mu);
+    pool_trim_locked(dev, keep);
 }
 
 constexpr size_t FIELD_SLACK = 256;
@@ -82,13 +93,14 @@ static int dmalloc(void** p, size_t bytes, const char* what) {
             const std::pair<void*, uint64_t> b = it->second.back();
             it->second.pop_back();
             pool_bytes -= bytes;
+            pool_dev_bytes[dev] -= bytes;
             pool_live[b.first] = {dev, bytes};
-            if (b.second > pool_synced) {  // freed since the last synchronization
+            if (b.second > pool_synced[dev]) {  // freed since the device's last synchronization
                 const uint64_t upto = pool_frees;
                 lk.unlock();
                 (void)hipDeviceSynchronize();
                 lk.lock();
-                pool_synced = std::max(pool_synced, upto);
+                pool_synced[dev] = std::max(pool_synced[dev], upto);
             }
             *p = b.first;
             return ADF_OK;
@@ -191,6 +203,7 @@ static void dfree(void* p) {
             if (pool_bytes + key.second <= POOL_MAX) {
                 pool_free[key].push_back({p, ++pool_frees});
                 pool_bytes += key.second;
+                pool_dev_bytes[key.first] += key.second;
                 return;
             }
         }
