@@ -37,7 +37,7 @@ EXPORTS = (
 ADF_REQ_QUEUED, ADF_REQ_RUNNING, ADF_REQ_DONE, ADF_REQ_FAILED = 0, 1, 2, -1
 ADF_SERVER_CHUNK, ADF_SERVER_BUDGET = 1, 2
 ADF_OPT_PATH, ADF_OPT_TILE_Y, ADF_OPT_BLOCK_Z, ADF_OPT_TIME_SWEEP, ADF_OPT_WAVES, ADF_OPT_BLOCK_Z_GEN = 1, 2, 3, 4, 5, 6
-ADF_OPT_PIPELINE, ADF_OPT_SWEEP_LAUNCH, ADF_OPT_STORE_NT, ADF_OPT_TX_LAUNCH = 7, 8, 9, 10
+ADF_OPT_PIPELINE, ADF_OPT_SWEEP_LAUNCH, ADF_OPT_STORE_NT, ADF_OPT_TX_LAUNCH, ADF_OPT_CELLS_PER_LANE = 7, 8, 9, 10, 11
 ADF_SWEEP_ONE_LAUNCH, ADF_SWEEP_TWO_STREAMS, ADF_SWEEP_TWO_LAUNCHES = 0, 1, 2
 ADF_BATCH_PERSISTENT, ADF_BATCH_GRAPH_STEPS = 1, 2
 ADF_BW_COPY_BUF, ADF_BW_COPY_GLOBAL, ADF_BW_READ, ADF_BW_WRITE, ADF_BW_COPY_NT = 0, 1, 2, 3, 4

@@ -27,7 +27,9 @@ _lock = threading.Lock()
 # slabs and boundary); pipeline: the prefetching tile routines; launch: one
 # launch over all tiles, or side and interior tiles as two launches on two
 # streams at the same time (single solves; ADF_SWEEP_*); store_nt: the
-# non-temporal store hint, tried on the best tiling of a single solve
+# non-temporal store hint, tried on the best tiling of a single solve; cpl:
+# cells per lane (ADF_OPT_CELLS_PER_LANE), 2 tried beside the default for
+# rows wider than 256 cells
 SOLVER_CANDIDATES = [dict(tile_y=0, block_z=kb, block_z_gen=kbg, waves=1, pipeline=pf, launch=ln)
                      for kb, kbg, pf, ln in itertools.product((4, 8, 16, 32), (2, 4), (1, 0), (0, 1))]
 BATCH_CANDIDATES = [dict(tile_y=0, block_z=kb, block_z_gen=kbg, waves=1, persistent=pers, pipeline=pf)
@@ -75,7 +77,7 @@ def _scratch(s: Solver) -> Solver:
 def _apply(s: Solver, shape: dict) -> None:
     s.set_path("auto", shape["tile_y"], shape["block_z"], shape["waves"], shape["block_z_gen"],
                pipeline=shape.get("pipeline", 1), launch=shape.get("launch", L.ADF_SWEEP_ONE_LAUNCH),
-               store_nt=shape.get("store_nt", 0))
+               store_nt=shape.get("store_nt", 0), cells_per_lane=shape.get("cpl", 0))
 
 
 def tune_solver(s: Solver, steps: int = 20, candidates: Optional[Sequence[dict]] = None,
@@ -93,7 +95,14 @@ def tune_solver(s: Solver, steps: int = 20, candidates: Optional[Sequence[dict]]
     x.set_option(L.ADF_OPT_TIME_SWEEP, 1)
     n = max(4, min(steps, s.dom.nt // 2))
     best, best_ms = None, float("inf")
-    for cand in candidates or SOLVER_CANDIDATES:
+    if candidates is None:
+        candidates = SOLVER_CANDIDATES
+        if s.dom.nx > 256:  # rows of several waves: 2 cells per lane fill more lanes per segment
+            candidates = candidates + [dict(c, cpl=2) for c in SOLVER_CANDIDATES]
+        tuned = True
+    else:
+        tuned = False
+    for cand in candidates:
         try:
             _apply(x, cand)
             x.reset()
@@ -104,7 +113,7 @@ def tune_solver(s: Solver, steps: int = 20, candidates: Optional[Sequence[dict]]
         ms, _ = x.sweep_stats()
         if ms < best_ms:
             best, best_ms = dict(cand), ms
-    if best is not None and best.get("pipeline", 1) and candidates is None:
+    if best is not None and best.get("pipeline", 1) and tuned:
         # the store hint on the best tiling, both timed again over more steps
         timed = {}
         m = max(1, min(2 * n, s.dom.nt - 2))

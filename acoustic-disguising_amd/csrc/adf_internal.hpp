@@ -114,6 +114,7 @@ struct adf_solver {
     int store_nt = 0;     // non-temporal stores in the pipelined routines
     int launch = 0;       // ADF_SWEEP_* (one launch, two streams, two serial)
     int tx_launch = 1;    // ADF_OPT_TX_LAUNCH
+    int cpl_opt = 0;      // ADF_OPT_CELLS_PER_LANE (0 auto)
 
     // GF extrapolation map
     int gf_kind = 0;

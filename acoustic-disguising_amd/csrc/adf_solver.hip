@@ -842,6 +842,11 @@ int solver_set_option(adf_solver* s, int option, int64_t value) {
         if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_PIPELINE must be 0 or 1");
         s->pipeline = (int)value;
         touch(s);
+    } else if (option == ADF_OPT_CELLS_PER_LANE) {
+        if (value != 0 && value != 1 && value != 2 && value != 4)
+            return fail(ADF_EINVAL, "ADF_OPT_CELLS_PER_LANE must be 0, 1, 2 or 4");
+        s->cpl_opt = (int)value;
+        touch(s);
     } else if (option == ADF_OPT_TX_LAUNCH) {
         if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_TX_LAUNCH must be 0 or 1");
         s->tx_launch = (int)value;
@@ -893,7 +898,7 @@ static int fused_xseg(int64_t nx, int cpl) {
 // 4 GiB, and a tile entry holds at most 63 row segments.
 static int fused_cpl(const adf_solver* s) {
     const int64_t nx = s->dom.nx;
-    const int cpl = nx > 128 ? 4 : (nx > 64 ? 2 : 1);
+    const int cpl = s->cpl_opt ? s->cpl_opt : (nx > 128 ? 4 : (nx > 64 ? 2 : 1));
     if (fused_nseg(nx, cpl) > 63) return 0;
     if (4 * s->n3 * (int64_t)s->es >= (int64_t(1) << 32)) return 0;
     return cpl;

@@ -236,6 +236,12 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
  *                        and one for the velocity fix-up.  Results are
  *                        identical. */
 #define ADF_OPT_TX_LAUNCH 10
+/*   ADF_OPT_CELLS_PER_LANE 0 (default): the fused kernel's lanes hold 1
+ *                        cell up to nx = 64, 2 up to 128, 4 above; or 1, 2
+ *                        or 4.  Rows wider than 64 lanes are cut into
+ *                        segments of at most 62 lanes, one wave each.
+ *                        Results are identical. */
+#define ADF_OPT_CELLS_PER_LANE 11
 int adf_solver_set_option(adf_solver* s, int option, int64_t value);
 
 /* Mean device time (ms) of one field-update launch and the number of

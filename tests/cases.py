@@ -74,12 +74,13 @@ def oracle_solver(O, d, txs, cpml, mask, mode, field=None):
 
 
 def gpu_run(d, txs, cpml, mask, dtype, precision, field=None, gf=None, path="auto", tile_y=0, block_z=0, waves=0,
-            pipeline=1, launch=0, store_nt=0, tx_launch=1):
+            pipeline=1, launch=0, store_nt=0, tx_launch=1, cells_per_lane=0):
     g = A.transceivers_to_grid(d, txs, dtype)
     f = np.zeros((4, d.nz, d.ny, d.nx), dtype) if field is None else np.array(field, dtype)
     cp = A.make_cpml(d, cpml.n, cpml.rcoef, cpml.fc, dtype)
     A.run_fdtd(d, f, g, cp, mask, gf_map=gf, precision=precision, path=path, tile_y=tile_y, block_z=block_z,
-               waves=waves, pipeline=pipeline, launch=launch, store_nt=store_nt, tx_launch=tx_launch)
+               waves=waves, pipeline=pipeline, launch=launch, store_nt=store_nt, tx_launch=tx_launch,
+               cells_per_lane=cells_per_lane)
     return f, g, cp
 
 

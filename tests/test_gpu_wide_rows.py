@@ -24,10 +24,12 @@ pytestmark = pytest.mark.gpu
 H = 0.00686
 TOL = {"f64": 0.0, "ref": 1e-6, "fast": 1e-5}
 
-# (path, tile_y, block_z, waves, pipeline, launch, store_nt): the pipelined
-# routines (auto tiling), the plane-at-a-time routines, and the pipelined
-# ones with 2 waves per workgroup, two launches and non-temporal stores
-PATHS = [("fused", 0, 0, 0, 1, 0, 0), ("fused", 0, 3, 1, 0, 0, 0), ("fused", 0, 4, 2, 1, 1, 1)]
+# (path, tile_y, block_z, waves, pipeline, launch, store_nt, cells per
+# lane): the pipelined routines (auto tiling), the plane-at-a-time routines,
+# the pipelined ones with 2 waves per workgroup, two launches and
+# non-temporal stores, and 2 and 1 cells per lane (more segments per row)
+PATHS = [("fused", 0, 0, 0, 1, 0, 0, 0), ("fused", 0, 3, 1, 0, 0, 0, 0), ("fused", 0, 4, 2, 1, 1, 1, 0),
+         ("fused", 0, 0, 0, 1, 0, 0, 2), ("fused", 0, 3, 1, 1, 0, 0, 1)]
 
 # 97: 2 cells per lane, ragged; 130: 4 per lane, ragged, one segment;
 # 257, 301: two segments, ragged; 520: three segments of 176 cells
@@ -74,7 +76,8 @@ def test_wide_and_ragged_rows_match_oracle(nx, kind):
     two = _arrays(*gpu_run(d, txs, cpml, mask, dtype, prec, field=field, path="two-pass"))
     for path in PATHS:
         f, g, cp = gpu_run(d, txs, cpml, mask, dtype, prec, field=field, path=path[0], tile_y=path[1],
-                           block_z=path[2], waves=path[3], pipeline=path[4], launch=path[5], store_nt=path[6])
+                           block_z=path[2], waves=path[3], pipeline=path[4], launch=path[5], store_nt=path[6],
+                           cells_per_lane=path[7])
         errs = {"P": rel_l2(f[0], ref.field[0]), "V": rel_l2(f[1:], ref.field[1:]),
                 "rec": max(rel_l2(g.rec[c], ref.tx["rec"][c]) for c in range(4)),
                 "vn_rec": rel_l2(g.vn_rec, ref.tx["vn_rec"]), "src": rel_l2(g.src, ref.tx["src"]),
@@ -137,9 +140,9 @@ def test_server_takes_a_paper_width_request():
         assert np.array_equal(sv.field(rid), want[1])
 
 
-@pytest.mark.parametrize("n,dtype,prec", [(301, np.float64, "fast"), (501, np.float32, "ref"),
-                                          (545, np.float32, "ref")])
-def test_paper_grids_fused_equals_two_pass(n, dtype, prec):
+@pytest.mark.parametrize("n,dtype,prec,cpl", [(301, np.float64, "fast", 0), (301, np.float32, "ref", 2),
+                                              (501, np.float32, "ref", 0), (545, np.float32, "ref", 0)])
+def test_paper_grids_fused_equals_two_pass(n, dtype, prec, cpl):
     """Full paper grids, a few steps from a random field with CPML and a
     sphere: the fused sweep (several segments, ragged rows) equals the
     two-pass kernels bit for bit (both are bit-exact to the oracle on the
@@ -159,7 +162,7 @@ def test_paper_grids_fused_equals_two_pass(n, dtype, prec):
         s.build_mask("sphere", [20 * H])
         s.set_field(f0)
         s.set_transceivers(A.transceivers_to_grid(d, txs, dtype))
-        s.set_path(path)
+        s.set_path(path, cells_per_lane=cpl)
         s.run(1, d.nt)
         out.append(_state(s))
         del s
