@@ -295,19 +295,14 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
     const uint32_t mfill = (masked && !NOMASK) ? 0u : 0xffffffffu;  // all open without a mask
     const int vl = i0c * ES;
     const int vs = lane_ok ? i0 * ES : (int)OOR;
-    // per-channel lane offsets (the channel's start in the vector offset,
-    // so each row needs one scalar offset for all four channels)
-    int vlc[4], vsc[4], vmc[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        vlc[c] = (int)((uint32_t)vl + (uint32_t)c * chb);
-        vsc[c] = lane_ok ? (int)((uint32_t)vs + (uint32_t)c * chb) : (int)OOR;
-    }
+    // one lane offset for the four field channels and one for the mask
+    // planes: the channel's start goes into the scalar offset with the
+    // row's (the range check that drops the out-of-range stores covers the
+    // vector offset only)
     const int vm = (i0c >> 5) * 4;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) vmc[c] = vm + c * (nz * ny * F.nwx) * 4;
     const int sh = i0c & 31;
     auto rowb = [&](int j, int k) { return (int)((uint32_t)(nx * j + sxy * k) * (uint32_t)ES); };
+    auto chs = [&](int c, int rb) { return (int)((uint32_t)rb + (uint32_t)c * chb); };
     auto maskb = [&](int plane, int j, int k) { return (((plane * nz + k) * ny + j) * F.nwx) * 4; };
 
     const XSlab<T, CPL> XP = xslab<T, CPL>(F, i0, p, 0, nx - 1, cpP, 0, 2 * p, F.mv_off(0));
@@ -355,7 +350,7 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
     T vzc[TY + 1][CPL];
     T p1prev[TY + 1][CPL];
 #pragma unroll
-    for (int r = 0; r <= TY; ++r) bld<T, CPL>(rF0, vlc[3], rowb(j0 + r - 1, k0 - 1), vzc[r]);
+    for (int r = 0; r <= TY; ++r) bld<T, CPL>(rF0, vl, chs(3, rowb(j0 + r - 1, k0 - 1)), vzc[r]);
 
     auto load = [&](Plane& L, int k) {
         if constexpr (SZ) {
@@ -376,11 +371,11 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
         for (int r = 0; r <= TY + 1; ++r) {
             const int j = j0 + r - 1;
             const int rb = rowb(j, k);
-            bld<T, CPL>(rF0, vlc[2], rb, L.vy0[r]);
+            bld<T, CPL>(rF0, vl, chs(2, rb), L.vy0[r]);
             if (r > TY) continue;
-            bld<T, CPL>(rF0, vlc[0], rb, L.p0[r]);
-            bld<T, CPL>(rF0, vlc[1], rb, L.vx0[r]);
-            bld<T, CPL>(rF0, vlc[3], rowb(j, k + 1), L.vzn[r]);
+            bld<T, CPL>(rF0, vl, rb, L.p0[r]);
+            bld<T, CPL>(rF0, vl, chs(1, rb), L.vx0[r]);
+            bld<T, CPL>(rF0, vl, chs(3, rowb(j, k + 1)), L.vzn[r]);
             bld<T, CPL>(rSP0, vP, p * (j + ny * k) * ES, L.mPx[r]);
             if constexpr (SY)
                 bld<T, CPL>(rsrc_of(F.psi0, yinP[r] ? psib : 0u), vl, yoP[r] + k * ypP, L.mPy[r]);
@@ -402,11 +397,11 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
                 L.mp_[r] = L.mvx_[r] = L.mvy_[r] = L.mvz_[r] = 0u;
                 continue;
             }
-            L.mp_[r] = bld_u32(rM, vmc[0], maskb(0, j, k));
+            L.mp_[r] = bld_u32(rM, vm, maskb(0, j, k));
             if (r > 0) {
-                L.mvx_[r] = bld_u32(rM, vmc[1], maskb(0, j, k));
-                L.mvy_[r] = bld_u32(rM, vmc[2], maskb(0, j, k));
-                L.mvz_[r] = bld_u32(rM, vmc[3], maskb(0, j, k));
+                L.mvx_[r] = bld_u32(rM, vm, maskb(1, j, k));
+                L.mvy_[r] = bld_u32(rM, vm, maskb(2, j, k));
+                L.mvz_[r] = bld_u32(rM, vm, maskb(3, j, k));
             }
         }
     };
@@ -477,10 +472,10 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
                 if constexpr (SZ) dpz = M_::psi(L.mVz[r][q], L.bzV, L.azV, dpz);
                 vz1[q] = M_::upd(vzc[r][q], F.fv, F.fvf, (mvz >> q) & 1u, dpz);
             }
-            bst<T, CPL, POL>(rF1, vsc[0], rb, p1[r]);
-            bst<T, CPL, POL>(rF1, vsc[1], rb, vx1);
-            bst<T, CPL, POL>(rF1, vsc[2], rb, vy1);
-            bst<T, CPL, POL>(rF1, vsc[3], rb, vz1);
+            bst<T, CPL, POL>(rF1, vs, rb, p1[r]);
+            bst<T, CPL, POL>(rF1, vs, chs(1, rb), vx1);
+            bst<T, CPL, POL>(rF1, vs, chs(2, rb), vy1);
+            bst<T, CPL, POL>(rF1, vs, chs(3, rb), vz1);
             if constexpr (RAG) {
 #pragma unroll
                 for (int q = 0; q < CPL - 1; ++q) {
