@@ -273,9 +273,10 @@ def cpu_sample_steps(nt, one_step_s, max_seconds):
     return int(max(2, min(nt - 2, max_seconds / max(one_step_s, 1e-3))))
 
 
-def cpu_leg(d, txs, p, max_seconds=15.0):
-    """The oracle in the reference's own Float32 arithmetic, OpenMP over k,
-    on a bounded sample: consecutive steps of the same 256^3 workload."""
+def _cpu_steps(d, txs, p, sphere, mode, max_seconds):
+    """The oracle (OpenMP over k on every physical core) stepping the given
+    solve: (steps timed, seconds, threads, host info), on a bounded sample of
+    consecutive steps after two warm-up steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the checker / CPU baseline only
     import adfdtd as A
@@ -283,12 +284,13 @@ def cpu_leg(d, txs, p, max_seconds=15.0):
     info = host_cpu()
     threads = cpu_threads(info)
     O.set_threads(threads)
+    T = np.float64 if mode == "f64" else np.float32
     rd = O.domain(d.tmax, d.xmax, d.ymax, d.zmax, d.nx, d.ny, d.nz, d.c0, d.r0, d.cf)
-    g = A.transceivers_to_grid(d, txs, np.float32)
+    g = A.transceivers_to_grid(d, txs, T)
     tx = {"idx": g.idx, "w": g.w, "src": g.src, "rec": g.rec, "dir": g.direction, "vn_src": g.vn_src,
           "vn_rec": g.vn_rec}
-    mask = A.build_update_mask(d, "sphere", [24 * d.dx])
-    s = O.RefSolver(rd, npml=p, rcoef=1e-4, fc=FC, mask=mask, txs=tx, mode="r32")
+    mask = A.build_update_mask(d, "sphere", [sphere])
+    s = O.RefSolver(rd, npml=p, rcoef=1e-4, fc=FC, mask=mask, txs=tx, mode=mode)
     s.run(1, 1)
     t0 = time.perf_counter()
     s.run(2, 2)
@@ -296,11 +298,38 @@ def cpu_leg(d, txs, p, max_seconds=15.0):
     k = cpu_sample_steps(d.nt, one, max_seconds)
     t0 = time.perf_counter()
     s.run(3, 2 + k)
-    el = time.perf_counter() - t0
+    return k, time.perf_counter() - t0, threads, info
+
+
+def cpu_leg(d, txs, p, mode="r32", max_seconds=15.0):
+    """The oracle in the reference's own Float32 arithmetic (or Float64),
+    OpenMP over k, on a bounded sample: consecutive steps of the same 256^3
+    workload."""
+    k, el, threads, info = _cpu_steps(d, txs, p, 24 * d.dx, mode, max_seconds)
+    arith = "Float64" if mode == "f64" else "the reference's Float32"
     return {"value": d.nx * d.ny * d.nz * k / el / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
-            "sample": f"{k} consecutive steps (3..{2 + k}) of the same 256^3 workload, oracle in the reference's "
-                      f"Float32 arithmetic, OpenMP over k on {threads} threads, {el:.1f} s",
+            "sample": f"{k} consecutive steps (3..{2 + k}) of the same 256^3 workload, oracle in {arith} "
+                      f"arithmetic, OpenMP over k on {threads} threads, {el:.1f} s",
             "host": info}
+
+
+def cpu_batch_leg(nt, max_seconds=10.0):
+    """The batched workload on the host: its solves one after another, each
+    on every physical core (oracle, the reference's Float32 arithmetic),
+    from a bounded sample of one 128^3 solve's steps."""
+    sp = batch_specs(64, [128], nt)[0]
+    d = sp["d"]
+    z = np.zeros(d.nt)
+    import adfdtd as A
+
+    txs = [A.make_transceiver(d, sp["src"], [1.0, 0.0, 0.0, 0.0], sp["tf"])]
+    txs += [A.make_transceiver(d, sp["rec"][i], np.r_[0.0, sp["rec_n"][i]], z) for i in range(len(sp["rec"]))]
+    k, el, threads, info = _cpu_steps(d, txs, sp["npml"], sp["sphere"], "r32", max_seconds)
+    per_solve = el / k * nt
+    return {"value": 1.0 / per_solve, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{k} consecutive steps of one 128^3 solve of the batched workload ({nt} steps per solve), "
+                      f"oracle in the reference's Float32 arithmetic on {threads} threads, {el:.1f} s; solves "
+                      f"run one after another"}
 
 
 def calibrate_copy(kinds=None, sizes=(1 << 28, 1 << 30), reps=20):
@@ -792,7 +821,10 @@ def main():
     if not args.skip_cpu:
         d2, txs, p2 = build_workload(nt=4000)
         res["cpu_baseline"] = cpu_leg(d2, txs, p2)
-        log("CPU baseline leg done")
+        res["cpu_baseline_f64"] = cpu_leg(d2, txs, p2, mode="f64", max_seconds=10.0)
+        if "batched" in res:
+            res["batched"]["cpu_baseline"] = cpu_batch_leg(args.batch_steps)
+        log("CPU baseline legs done")
     print(json.dumps(res))
 
 
