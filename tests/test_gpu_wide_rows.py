@@ -191,3 +191,25 @@ def test_paper_grid_matches_oracle(n, kind):
             assert np.array_equal(a, b), q
         else:
             assert rel_l2(a, b) <= 1e-6, (q, rel_l2(a, b))
+
+
+@pytest.mark.parametrize("nx", [3, 5, 7, 9, 13, 31, 33, 63, 65, 127, 129, 131])
+def test_narrow_and_odd_rows_at_every_cells_per_lane(nx):
+    """Rows narrower than a lane's cells (nx = 3 at 4 cells per lane; the
+    library's smallest grid is 3^3),
+    odd and one-past-a-boundary widths, at 1, 2 and 4 cells per lane and
+    the automatic choice: Float64 bit for bit against the oracle, random
+    field, CPML where the row has room for it."""
+    p = min(3, (nx - 1) // 3)
+    d, txs, cpml, mask = build(n=(nx, 14, 13), nt=6, npml=p, n_rec=0, src_cells=(0.2 - (nx % 2) * 0.5, 0.3, -0.4),
+                               src_dir=(1.0, 0.3, -0.2, 0.1))
+    f0 = np.random.default_rng(nx).standard_normal((4, d.nz, d.ny, d.nx))
+    ref = oracle_solver(O, d, txs, cpml, mask, "f64", field=f0).run()
+    want = [ref.field, ref.tx["src"], ref.tx["vn_rec"]] + list(ref.tx["rec"]) + list(ref.mp) + list(ref.mv)
+    for cpl in (0, 1, 2, 4):
+        for pf in (1, 0):
+            f, g, cp = gpu_run(d, txs, cpml, mask, np.float64, "fast", field=f0, path="fused", block_z=2,
+                               pipeline=pf, cells_per_lane=cpl)
+            got = [f, g.src, g.vn_rec] + list(g.rec) + list(cp.p) + list(cp.v)
+            for q, (a, b) in enumerate(zip(got, want)):
+                assert np.array_equal(a, b), (cpl, pf, q)
