@@ -33,7 +33,7 @@ _lock = threading.Lock()
 SOLVER_CANDIDATES = [dict(tile_y=0, block_z=kb, block_z_gen=kbg, waves=1, pipeline=pf, launch=ln)
                      for kb, kbg, pf, ln in itertools.product((4, 8, 16, 32), (2, 4), (1, 0), (0, 1))]
 BATCH_CANDIDATES = [dict(tile_y=0, block_z=kb, block_z_gen=kbg, waves=1, persistent=pers, pipeline=pf)
-                    for kb, kbg, pers, pf in itertools.product((4, 8, 16), (2, 4), (0, 1), (1, 0))]
+                    for kb, kbg, pers, pf in itertools.product((4, 8, 16), (2, 4, 8), (0, 1), (1, 0))]
 
 
 def cache_path() -> str:
