@@ -178,6 +178,7 @@ extern "C" int adf_solver_set_mask(adf_solver* s, const uint8_t* m) {
     if (!s) return fail(ADF_EINVAL, "adf_solver_set_mask: null solver");
     HIPCHK(hipSetDevice(s->ctx->device));
     touch(s);
+    s->open_ok = false;
     if (!m) {
         s->has_mask = false;
         return ADF_OK;
@@ -203,6 +204,7 @@ extern "C" int adf_solver_build_mask(adf_solver* s, int shape, const double* par
                                      const double* yc, const double* zc) {
     if (!s) return fail(ADF_EINVAL, "adf_solver_build_mask: null solver");
     touch(s);
+    s->open_ok = false;
     if (shape == 0) {
         s->has_mask = false;
         return ADF_OK;

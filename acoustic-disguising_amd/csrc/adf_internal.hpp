@@ -66,6 +66,10 @@ struct adf_solver {
     // 4 mask bit-planes
     uint32_t* mask = nullptr;
     bool has_mask = false;
+    // rows (j,k) whose four mask planes are all open, kept from the last
+    // scan of the mask; cleared by every mask change
+    std::vector<uint8_t> open_h;
+    bool open_ok = false;
 
     // transceivers (device copies; traces in the reference (nt,ntx) layout)
     int64_t ntx = 0;
@@ -78,6 +82,7 @@ struct adf_solver {
     void* tx_vn_rec = nullptr;
     uint32_t* order_p = nullptr;  // entries: index | injector << 31
     uint32_t* order_v = nullptr;
+    std::vector<uint32_t> h_order_p, h_order_v;  // host copies of order_p, order_v
     std::vector<int> lev_p;  // level offsets into order_p (size nlev+1)
     // host copies for rebuilding the schedule: 1-based int32 idx [4][ntx][3]
     // and per-transceiver injector flags by source buffer
@@ -88,6 +93,7 @@ struct adf_solver {
     // velocity cells next to pressure-injection stencils (fused path)
     uint32_t* fix_cells = nullptr;
     int n_fix = 0;
+    std::vector<uint32_t> h_fix;  // host copy of fix_cells
     // one-launch transceiver step (k_tx_step): the chain blocks' lists
     // followed by the record-only indices (from txc_rec_off), and the chain
     // blocks' offsets (TXC_HDR ints each)

@@ -219,6 +219,9 @@ static void free_txs(adf_solver* s) {
     s->txc_hdr = nullptr;
     s->fix_cells = nullptr;
     s->n_fix = 0;
+    s->h_fix.clear();
+    s->h_order_p.clear();
+    s->h_order_v.clear();
     s->txc_list = nullptr;
     s->txc_ok = false;
     s->tx_idx = nullptr; s->tx_w = s->tx_src = s->tx_rec = s->tx_dir = s->tx_vn_src = s->tx_vn_rec = nullptr;
@@ -535,6 +538,7 @@ int rebuild_schedule(adf_solver* s) {
     const int cp[1] = {0};
     make_levels(ntx, nx, ny, nz, s->h_idx, ip, cp, 1, order, s->lev_p);
     HIPCHK(hipMemcpyAsync(s->order_p, order.data(), (size_t)ntx * 4, hipMemcpyHostToDevice, st));
+    s->h_order_p = order;
     std::vector<uint32_t> order_v;
     const int cv[3] = {1, 2, 3};
     make_levels(ntx, nx, ny, nz, s->h_idx, iv, cv, 3, order_v, s->lev_v);
@@ -563,6 +567,7 @@ int rebuild_schedule(adf_solver* s) {
         HIPCHK(hipMemcpy(s->gf_slot_next, next.data(), (size_t)nr * 4, hipMemcpyHostToDevice));
     }
     HIPCHK(hipMemcpyAsync(s->order_v, order_v.data(), (size_t)ntx * 4, hipMemcpyHostToDevice, st));
+    s->h_order_v = order_v;
 
     // Velocity cells whose update reads a pressure cell an injector writes:
     // the fused pass recomputes them after the injection (k_fixup_v).
@@ -588,11 +593,14 @@ int rebuild_schedule(adf_solver* s) {
     HIPCHK(hipStreamSynchronize(st));
     dfree(s->fix_cells);
     s->fix_cells = nullptr;
-    s->n_fix = (int)cells.size();
-    if (s->n_fix) {
+    s->n_fix = 0;
+    s->h_fix.clear();
+    if (!cells.empty()) {
         int rc = dmalloc((void**)&s->fix_cells, cells.size() * 4, "fixup list");
         if (rc) return rc;
         HIPCHK(hipMemcpy(s->fix_cells, cells.data(), cells.size() * 4, hipMemcpyHostToDevice));
+        s->n_fix = (int)cells.size();
+        s->h_fix = cells;
     }
 
     // The mirrors can ride inside the V update unless a velocity stencil
@@ -1054,10 +1062,14 @@ static void launch_after_v(adf_solver* s, const Solve<T>& S, const Txs<T>& X, in
 // entries as xcd_tile describes, for member `m`.
 // Rows (j, k) whose cells are all fluid with all faces open (every row
 // when the solve has no mask), from the device mask.
-static int open_rows(const adf_solver* s, std::vector<uint8_t>& open) {
+static int open_rows(adf_solver* s, std::vector<uint8_t>& open) {
     const int64_t nx = s->dom.nx, ny = s->dom.ny, nz = s->dom.nz, nwx = s->nwx;
     open.assign((size_t)(ny * nz), 1);
     if (!s->has_mask) return ADF_OK;
+    if (s->open_ok) {
+        open = s->open_h;
+        return ADF_OK;
+    }
     std::vector<uint32_t> bits((size_t)(4 * nwx * ny * nz));
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     HIPCHK(hipMemcpy(bits.data(), s->mask, bits.size() * 4, hipMemcpyDeviceToHost));
@@ -1070,6 +1082,8 @@ static int open_rows(const adf_solver* s, std::vector<uint8_t>& open) {
                 if ((w[q] & want) != want) { open[(size_t)r] = 0; break; }
             }
         }
+    s->open_h = open;
+    s->open_ok = true;
     return ADF_OK;
 }
 
@@ -1602,9 +1616,10 @@ static int batch_build(adf_batch* b) {
         adf_solver* s = b->members[m];
         if (!s->fuse_mirror) mir.push_back(m);
         if (!s->ntx) continue;
-        std::vector<uint32_t> op(s->ntx), ov(s->ntx);
-        HIPCHK(hipMemcpy(op.data(), s->order_p, s->ntx * 4, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(ov.data(), s->order_v, s->ntx * 4, hipMemcpyDeviceToHost));
+        const std::vector<uint32_t>& op = s->h_order_p;
+        const std::vector<uint32_t>& ov = s->h_order_v;
+        if ((int64_t)op.size() != s->ntx || (int64_t)ov.size() != s->ntx)
+            return fail(ADF_ESTATE, "adf_batch_run: a member's transceiver schedule is missing");
         auto merge = [&](const std::vector<uint32_t>& o, const std::vector<int>& lev, std::vector<std::vector<uint32_t>>& out) {
             if (out.size() + 1 < lev.size()) out.resize(lev.size() - 1);
             for (size_t L = 0; L + 1 < lev.size(); ++L)
@@ -1615,11 +1630,7 @@ static int batch_build(adf_batch* b) {
         };
         merge(op, s->lev_p, lp);
         merge(ov, s->lev_v, lv);
-        if (s->n_fix) {
-            std::vector<uint32_t> fc(s->n_fix);
-            HIPCHK(hipMemcpy(fc.data(), s->fix_cells, s->n_fix * 4, hipMemcpyDeviceToHost));
-            for (uint32_t c : fc) fix.push_back(make_uint2((uint32_t)m, c));
-        }
+        for (uint32_t c : s->h_fix) fix.push_back(make_uint2((uint32_t)m, c));
     }
     auto upload_levels = [&](const std::vector<std::vector<uint32_t>>& lv_, uint32_t** dst, std::vector<int>& off) {
         std::vector<uint32_t> flat;

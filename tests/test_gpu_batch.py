@@ -161,3 +161,41 @@ def test_batch_errors():
     wide = A.Solver(d, L.ADF_F32, None)
     with pytest.raises(L.AdfError, match="row segments"):
         b.add(wide)
+
+
+def test_mask_and_transceivers_changed_mid_run():
+    """A member whose mask and transceivers change between batch runs takes
+    the new ones in its next step; a solo fused solver does the same.  In
+    Float64 both equal the two-pass kernels, which read the mask cell by
+    cell, bit for bit."""
+    h = 0.00686
+    kw = dict(n=(48, 20, 22), nt=60, npml=6, n_rec=8, rec_radius_cells=6)
+    d, txs, cpml, m1 = build(shape="sphere", shape_params=[3 * h], **kw)
+    m2 = build(shape="cube", shape_params=[7 * h], **kw)[3]
+    txs2 = build(src_cells=(-2.6, 1.3, 0.8), **kw)[1]
+    other = cases()[1]
+    runs = []
+    for mode in ("two-pass", "fused", "batch"):
+        s = make_solver((d, txs, cpml, m1), np.float64)
+        if mode != "batch":
+            s.set_path(mode)
+            runs.append((s, None))
+            continue
+        b = A.Batch(graph_steps=8)
+        o = make_solver(other, np.float64)
+        b.add(s)
+        b.add(o)
+        runs.append((s, (b, o)))
+    for s, bo in runs:
+        step = (lambda n: bo[0].run(n)) if bo else (lambda n, s=s: s.run(s.next_step(), s.next_step() + n - 1))
+        step(20)
+        s.set_mask(m2)
+        step(20)
+        s.set_transceivers(A.transceivers_to_grid(d, txs2, np.float64))
+        step(d.nt - 40)
+        assert s.next_step() == d.nt + 1
+    assert_same(runs[1][0], runs[0][0], "fused solo")
+    assert_same(runs[2][0], runs[0][0], "batch member")
+    solo = make_solver(other, np.float64)
+    solo.run(1, d.nt)
+    assert_same(runs[2][1][1], solo, "unchanged member")
