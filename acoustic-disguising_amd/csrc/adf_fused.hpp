@@ -365,6 +365,14 @@ struct FA {
             return dv + (float)m;
         }
     }
+    // The same where a cell with `keep` set keeps its memory (a select, so
+    // the memory comes back bit for bit, -0.0 included)
+    static __device__ __forceinline__ A psi_keep(T& m, T bc, T ac, A dv, bool keep) {
+        const T m0 = m;
+        const A d = psi(m, bc, ac, dv);
+        if (keep) m = m0;
+        return d;
+    }
     // x1 = x0 - f*s (f = 0 where the mask closes the cell)
     static __device__ __forceinline__ T upd(T x0, double f, float ff, bool open, A s) {
         if constexpr (EXACT) return (T)((double)x0 - (open ? f : 0.0) * s);

@@ -20,20 +20,22 @@ namespace adf {
 
 // Lane constants of the x direction for one phase: per-cell coefficients
 // (zero outside the slab), the lane's element offset into the phase's
-// x-slab CPML array (side and in-row position of its first cell) and the
-// bitmask of its cells inside the slab.
+// x-slab CPML array (side and in-row position of its first cell), the
+// bitmask of its cells inside the slab and the bitmask of its cells whose
+// position lies inside the slab row's extent [0, w) (a lane at the edge of
+// the slab may reach into the neighbouring row's memory).
 template <typename T, int CPL>
 struct XSlab {
     T a[CPL], b[CPL];
     int off;
-    uint32_t in;
+    uint32_t in, ext;
 };
 
 // Phase with slab width w (p for P, p+1 for V) over cells [lo, hi) of the
-// row; `base` is the array's element offset in psi0/psi1.
+// row; `base` is the array's element offset in psi0/psi1.  carry: see below.
 template <typename T, int CPL>
 __device__ __forceinline__ XSlab<T, CPL> xslab(const Fused<T>& F, int i0, int w, int lo, int hi, bool on,
-                                               int coef_a, int coef_b, int base) {
+                                               int coef_a, int coef_b, int base, bool carry = false) {
     XSlab<T, CPL> X;
     X.in = 0;
     X.off = base;
@@ -53,6 +55,20 @@ __device__ __forceinline__ XSlab<T, CPL> xslab(const Fused<T>& F, int i0, int w,
     // element offset of cell i0 (may point before/after the slab row for
     // cells outside it; those lanes only load, never store)
     if (side) X.off = base + l0 + w * F.ny * F.nz * (side - 1);
+    // carry: a cell inside the extent but outside [lo, hi) (P at i = nx-1,
+    // Vx at i = 0: never updated) gets b = 1, a = 0, so its memory passes
+    // through the update unchanged (b m + 0 dv; dv there is a finite field
+    // difference, and the cell keeps its field value, so dv + psi is not
+    // used) and a vector store writes it back as loaded, without a select.
+    // The exact routines keep it with a select instead (m + 0 would turn a
+    // memory of -0.0 into +0.0).
+    X.ext = 0;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q)
+        if (side && l0 + q >= 0 && l0 + q < w) {
+            X.ext |= 1u << q;
+            if (carry && !((X.in >> q) & 1u)) X.b[q] = (T)1;
+        }
     return X;
 }
 
@@ -305,21 +321,24 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
     auto chs = [&](int c, int rb) { return (int)((uint32_t)rb + (uint32_t)c * chb); };
     auto maskb = [&](int plane, int j, int k) { return (((plane * nz + k) * ny + j) * F.nwx) * 4; };
 
-    const XSlab<T, CPL> XP = xslab<T, CPL>(F, i0, p, 0, nx - 1, cpP, 0, 2 * p, F.mv_off(0));
-    const XSlab<T, CPL> XV = xslab<T, CPL>(F, i0, qv, 1, nx, cpV, 4 * p, 4 * p + 2 * qv, F.mp_off(0));
+    const XSlab<T, CPL> XP = xslab<T, CPL>(F, i0, p, 0, nx - 1, cpP, 0, 2 * p, F.mv_off(0), !EXACT);
+    const XSlab<T, CPL> XV = xslab<T, CPL>(F, i0, qv, 1, nx, cpV, 4 * p, 4 * p + 2 * qv, F.mp_off(0), !EXACT);
     const int vP = XP.off * ES, vV = XV.off * ES;
     constexpr uint32_t FULL = (1u << CPL) - 1u;
-    // CPML memory stores of the x-slab cells: one vector store for lanes
-    // whose cells all lie in the slab, one element store per cell for lanes
-    // partly in it; the other lanes' offsets are out of range
-    const bool stP = XL.own && cpP, stV = XL.own && cpV;
-    const int sPv = (stP && XP.in == FULL) ? vP : (int)OOR;
-    const int sVv = (stV && XV.in == FULL) ? vV : (int)OOR;
+    // CPML memory stores of the x-slab cells.  A lane with cells in the slab
+    // whose positions all lie in the slab row's extent stores one vector
+    // (its cells outside the update range carry their memory unchanged, see
+    // xslab).  A lane reaching past the row's extent (into the neighbouring
+    // row's memory) stores its updated cells one by one.  The other lanes' offsets are out of range (those stores
+    // are dropped).
+    const bool stP = XL.own && cpP && XP.in, stV = XL.own && cpV && XV.in;
+    const int sPv = (stP && XP.ext == FULL) ? vP : (int)OOR;
+    const int sVv = (stV && XV.ext == FULL) ? vV : (int)OOR;
     int sPq[CPL], sVq[CPL];
 #pragma unroll
     for (int q = 0; q < CPL; ++q) {
-        sPq[q] = (stP && XP.in != FULL && ((XP.in >> q) & 1u)) ? vP + q * ES : (int)OOR;
-        sVq[q] = (stV && XV.in != FULL && ((XV.in >> q) & 1u)) ? vV + q * ES : (int)OOR;
+        sPq[q] = (stP && XP.ext != FULL && ((XP.in >> q) & 1u)) ? vP + q * ES : (int)OOR;
+        sVq[q] = (stV && XV.ext != FULL && ((XV.in >> q) & 1u)) ? vV + q * ES : (int)OOR;
     }
 
     // y slab of each row (fixed for the tile): psi row offset at plane 0
@@ -415,11 +434,13 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
             for (int q = 0; q < CPL; ++q) {
                 const T vxn = q + 1 < CPL ? L.vx0[r][q + 1] : vxl;
                 A dvx = M_::diff(vxn, L.vx0[r][q], F.dx, F.rdx);
-                if (cpP) dvx = M_::psi(L.mPx[r][q], XP.b[q], XP.a[q], dvx);
                 // P at i = nx-1 is never updated (forward.jl:67-68): that
-                // cell keeps P and its y/z slab memory (the x slab excludes
-                // it through its range)
+                // cell keeps P and its slab memory (see xslab for the x slab)
                 const bool upd = i0 + q < nx - 1;
+                if (cpP) {
+                    if constexpr (EXACT) dvx = M_::psi_keep(L.mPx[r][q], XP.b[q], XP.a[q], dvx, !upd);
+                    else dvx = M_::psi(L.mPx[r][q], XP.b[q], XP.a[q], dvx);
+                }
                 A dvy = M_::diff(L.vy0[r + 1][q], L.vy0[r][q], F.dy, F.rdy);
                 if constexpr (SY) {
                     const T m0 = L.mPy[r][q];
@@ -462,7 +483,10 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
             for (int q = 0; q < CPL; ++q) {
                 const T pm = q > 0 ? p1[r][q - 1] : pml;
                 A dpx = M_::diff(p1[r][q], pm, F.dx, F.rdx);
-                if (cpV) dpx = M_::psi(L.mVx[r][q], XV.b[q], XV.a[q], dpx);
+                if (cpV) {
+                    if constexpr (EXACT) dpx = M_::psi_keep(L.mVx[r][q], XV.b[q], XV.a[q], dpx, i0 + q < 1);
+                    else dpx = M_::psi(L.mVx[r][q], XV.b[q], XV.a[q], dpx);
+                }
                 const T vx = M_::upd(L.vx0[r][q], F.fv, F.fvf, (mvx >> q) & 1u, dpx);
                 vx1[q] = (i0 + q >= 1) ? vx : L.vx0[r][q];
                 A dpy = M_::diff(p1[r][q], p1[r - 1][q], F.dy, F.rdy);
@@ -485,14 +509,17 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
                         bst<T, 1, POL>(rF1, vq[q], (int)((uint32_t)rb + (uint32_t)c * chb), o[c]);
                 }
             }
+            // x-slab memory: 64-byte pieces of a row, written with ordinary
+            // stores whatever POL is (with the non-temporal hint they cost
+            // 5 % of the 256^3 sweep: each piece then goes to HBM alone)
             const int soP = p * (j + ny * k) * ES, soV = qv * (j + ny * k) * ES;
-            bst<T, CPL, POL>(rS1, sPv, soP, L.mPx[r]);
-            bst<T, CPL, POL>(rS1, sVv, soV, L.mVx[r]);
+            bst<T, CPL>(rS1, sPv, soP, L.mPx[r]);
+            bst<T, CPL>(rS1, sVv, soV, L.mVx[r]);
 #pragma unroll
             for (int q = 0; q < CPL; ++q) {
                 const T a1[1] = {L.mPx[r][q]}, b1[1] = {L.mVx[r][q]};
-                bst<T, 1, POL>(rS1, sPq[q], soP, a1);
-                bst<T, 1, POL>(rS1, sVq[q], soV, b1);
+                bst<T, 1>(rS1, sPq[q], soP, a1);
+                bst<T, 1>(rS1, sVq[q], soV, b1);
             }
             // y/z slab memory (row vectors of the slab arrays)
             auto sslab = [&](Rsrc rs, int soff, const T (&o)[CPL]) {
