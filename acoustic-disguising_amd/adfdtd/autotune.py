@@ -27,7 +27,8 @@ _lock = threading.Lock()
 # slabs and boundary); pipeline: the prefetching tile routines; launch: one
 # launch over all tiles, or side and interior tiles as two launches on two
 # streams at the same time (single solves; ADF_SWEEP_*); store_nt: the
-# non-temporal store hint, tried on the best tiling of a single solve; cpl:
+# non-temporal store hint, tried on the best tiling of a single solve or a
+# batch; cpl:
 # cells per lane (ADF_OPT_CELLS_PER_LANE), 2 tried beside the default for
 # rows wider than 256 cells
 SOLVER_CANDIDATES = [dict(tile_y=0, block_z=kb, block_z_gen=kbg, waves=1, pipeline=pf, launch=ln)
@@ -179,6 +180,26 @@ def tune_batch(members: Sequence[Solver], steps: int = 24, candidates: Optional[
             del b
         if ms < best_ms:
             best, best_ms = dict(cand), ms
+    if best is not None and best.get("pipeline", 1) and candidates is None:
+        # the store hint on the best tiling, both timed again
+        timed = {}
+        for nt_ in (0, 1):
+            shape = dict(best, store_nt=nt_)
+            b = Batch(persistent=bool(shape["persistent"]), graph_steps=min(16, n))
+            try:
+                for x in scratch:
+                    x.reset()
+                    _apply(x, shape)
+                    b.add(x)
+                b.run(min(16, n))
+                b.run(n)
+                timed[nt_] = b.last_step_ms()
+            except L.AdfError:
+                timed[nt_] = float("inf")
+            finally:
+                del b
+        best["store_nt"] = int(timed[1] < timed[0])
+        best_ms = timed[best["store_nt"]]
     del scratch
     best = best or dict(BATCH_CANDIDATES[0])
     best["ms"] = best_ms

@@ -178,6 +178,7 @@ struct adf_batch {
     struct Group {
         int cpl, ty, waves, pf;  // pf: pipelined tile routines
         int rag;                 // rows not a multiple of cpl long
+        int store_nt;            // non-temporal field stores (pipelined routines)
         int merged;             // one launch over LIST_ALL, else LIST_SIDE then LIST_INT
         int n[3], grid[3];      // per tile list (LIST_* order)
         uint2* tiles[3];

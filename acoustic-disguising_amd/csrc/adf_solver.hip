@@ -1527,7 +1527,7 @@ static int batch_build(adf_batch* b) {
     std::vector<Fused<T>> fs(2 * n);
     std::vector<TxDev<T>> td(n);
     std::vector<int> it0(n), cur(n);
-    std::map<std::tuple<int, int, int, int, int, int>, std::array<std::vector<uint2>, TILE_LISTS>> glists;
+    std::map<std::tuple<int, int, int, int, int, int, int>, std::array<std::vector<uint2>, TILE_LISTS>> glists;
     // total work decides the z-block: enough tiles for every CU
     int64_t cells = 0;
     for (adf_solver* s : b->members) cells += s->n3;
@@ -1570,9 +1570,10 @@ static int batch_build(adf_batch* b) {
         std::vector<uint2> lists[TILE_LISTS];
         split_tiles(s, ty, kb, kbg, 1, (uint32_t)m, open, lists);
         const int cpl = fused_cpl(s);
-        const std::tuple<int, int, int, int, int, int> key(cpl, ty, s->waves ? s->waves : 1, s->pipeline,
-                                                           s->launch == ADF_SWEEP_ONE_LAUNCH ? 1 : 0,
-                                                           s->dom.nx % cpl != 0 ? 1 : 0);
+        const std::tuple<int, int, int, int, int, int, int> key(cpl, ty, s->waves ? s->waves : 1, s->pipeline,
+                                                                s->launch == ADF_SWEEP_ONE_LAUNCH ? 1 : 0,
+                                                                s->dom.nx % cpl != 0 ? 1 : 0,
+                                                                s->pipeline && s->store_nt ? 1 : 0);
         auto& G = glists[key];
         for (int c : {LIST_INT, LIST_SIDE}) G[c].insert(G[c].end(), lists[c].begin(), lists[c].end());
     }
@@ -1593,6 +1594,7 @@ static int batch_build(adf_batch* b) {
         g.pf = std::get<3>(kv.first);
         g.merged = std::get<4>(kv.first);
         g.rag = std::get<5>(kv.first);
+        g.store_nt = std::get<6>(kv.first);
         if (g.merged) merge_lists(kv.second[LIST_SIDE], kv.second[LIST_INT], g.waves, kv.second[LIST_ALL]);
         for (int h = 0; h < TILE_LISTS; ++h) {
             const std::vector<uint2>& L = kv.second[h];

@@ -75,6 +75,11 @@ static void batch_launch_list(adf_batch* b, const adf_batch::Group& g, hipStream
     const Fused<T>* fs = (const Fused<T>*)b->fs;
     const int pers = b->persistent ? 1 : 0;
     const dim3 blk(64 * g.waves);
+    if (g.pf && g.store_nt) {
+        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, true, RAG, ST_NT>), dim3(g.grid[LIST]), blk, 0, st, fs,
+                           b->cur, g.tiles[LIST], g.n[LIST], g.ctr[LIST], pers);
+        return;
+    }
     if (g.pf) {
         hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, true, RAG>), dim3(g.grid[LIST]), blk, 0, st, fs, b->cur,
                            g.tiles[LIST], g.n[LIST], g.ctr[LIST], pers);

@@ -222,10 +222,12 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
 #define ADF_SWEEP_TWO_STREAMS 1
 #define ADF_SWEEP_TWO_LAUNCHES 2
 /*   ADF_OPT_STORE_NT     1: the fused kernel's pipelined routines store
- *                        with the non-temporal hint (faster where the two
- *                        field buffers exceed the GPU's last-level cache,
- *                        e.g. 256^3; slower where they fit); 0 (default):
- *                        ordinary stores.  Results are identical. */
+ *                        field rows with the non-temporal hint (faster
+ *                        where the two field buffers exceed the GPU's
+ *                        last-level cache, e.g. 256^3; slower where they
+ *                        fit); 0 (default): ordinary stores.  A batch
+ *                        takes each member's setting.  Results are
+ *                        identical. */
 #define ADF_OPT_STORE_NT 9
 /*   ADF_OPT_TX_LAUNCH    1 (default): on the fused path, the transceivers
  *                        of a step run in one launch when the ones that

@@ -143,6 +143,25 @@ def test_tilings_do_not_change_results():
         assert_same(a, c, f"member {i}")
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_store_hint_does_not_change_results(dtype):
+    """Members with the non-temporal store hint (ADF_OPT_STORE_NT) sweep in
+    a launch of their own beside the others; every member still ends as its
+    solo run."""
+    cs = cases()
+    ref = [make_solver(c, dtype) for c in cs]
+    for s in ref:
+        s.run(1, s.dom.nt)
+    b = A.Batch(graph_steps=8)
+    mem = [make_solver(c, dtype) for c in cs]
+    for i, s in enumerate(mem):
+        s.set_path("auto", store_nt=i % 2)
+        b.add(s)
+    b.run(cs[0][0].nt)
+    for i, (a, c) in enumerate(zip(mem, ref)):
+        assert_same(a, c, f"member {i}")
+
+
 def test_batch_errors():
     cs = cases()
     b = A.Batch()
