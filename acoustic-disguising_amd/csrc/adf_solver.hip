@@ -820,7 +820,8 @@ int solver_set_option(adf_solver* s, int option, int64_t value) {
     if (option == ADF_OPT_PATH) {
         if (value < 0 || value > 2) return fail(ADF_EINVAL, "ADF_OPT_PATH must be 0, 1 or 2");
         if (value == 2 && !fused_supported(s))
-            return fail(ADF_EINVAL, "fused path needs 4 nx ny nz sizeof(T) < 4 GiB and at most 63 row segments");
+            return fail(ADF_EINVAL, "fused path needs 4 nx ny nz sizeof(T) < 4 GiB, at most 63 row segments and no "
+                                    "lane holding cells of both x CPML slabs");
         s->path = (int)value;
         touch(s);
     } else if (option == ADF_OPT_TILE_Y) {
@@ -901,14 +902,38 @@ static int fused_xseg(int64_t nx, int cpl) {
     return (int)(((nx + n - 1) / n + cpl - 1) / cpl * cpl);
 }
 
+// True when no lane of cpl cells (lanes start at multiples of cpl, also in
+// row segments) holds updated cells of both x slabs of a phase: the fused
+// routines address a lane's x-slab memory from the side of its first slab
+// cell.  Only rows whose two slabs (nearly) meet have such a lane.
+static bool x_slabs_apart(int64_t nx, int64_t p, int cpl) {
+    for (int ph = 0; ph < 2; ++ph) {
+        const int64_t w = ph ? p + 1 : p, lo = ph ? 1 : 0, hi = ph ? nx : nx - 1;  // V: [1, nx), P: [0, nx-1)
+        if (w <= 0) continue;
+        for (int64_t i0 = 0; i0 < nx; i0 += cpl) {
+            bool left = false, right = false;
+            for (int64_t i = i0; i < i0 + cpl && i < nx; ++i) {
+                if (i < lo || i >= hi) continue;
+                if (i < w) left = true;
+                else if (i >= nx - w) right = true;
+            }
+            if (left && right) return false;
+        }
+    }
+    return true;
+}
+
 // Cells per lane of the fused kernel.  It addresses a field buffer (4
 // channels) through one buffer descriptor, so it needs 4 n3 sizeof(T) <
-// 4 GiB, and a tile entry holds at most 63 row segments.
+// 4 GiB, a tile entry holds at most 63 row segments, and no lane may hold
+// cells of both x slabs (x_slabs_apart; such grids take the two-pass
+// kernels, whose slab test is per cell).
 static int fused_cpl(const adf_solver* s) {
     const int64_t nx = s->dom.nx;
     const int cpl = s->cpl_opt ? s->cpl_opt : (nx > 128 ? 4 : (nx > 64 ? 2 : 1));
     if (fused_nseg(nx, cpl) > 63) return 0;
     if (4 * s->n3 * (int64_t)s->es >= (int64_t(1) << 32)) return 0;
+    if (!x_slabs_apart(nx, s->p, cpl)) return 0;
     return cpl;
 }
 
@@ -1319,6 +1344,8 @@ static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int
 template <typename T, bool EXACT>
 static int run_impl(adf_solver* s, int64_t start, int64_t end) {
     hipStream_t st = s->ctx->stream;
+    if (s->path == 2 && !fused_supported(s))  // cells per lane changed after the fused path was chosen
+        return fail(ADF_EINVAL, "fused path not available for this grid at these cells per lane");
     const bool fused = s->path == 2 || (s->path == 0 && fused_supported(s));
     int ty = 2, kb = 16, kbg = 8;
     if (fused) {
@@ -1499,7 +1526,7 @@ int batch_add(adf_batch* b, adf_solver* s) {
     if (s->gf_kind) return fail(ADF_EINVAL, "adf_batch_add: solvers with a GF map run alone (adf_solver_run)");
     if (!fused_supported(s))
         return fail(ADF_EINVAL, "adf_batch_add: batched solves need the fused sweep (4 nx ny nz sizeof(T) < 4 GiB, "
-                                "at most 63 row segments)");
+                                "at most 63 row segments, no lane holding cells of both x CPML slabs)");
     if ((int64_t)b->members.size() >= 4096) return fail(ADF_EINVAL, "adf_batch_add: at most 4096 members");
     if (s->ntx >= (1 << 19)) return fail(ADF_EINVAL, "adf_batch_add: at most 524287 transceivers per member");
     if (b->members.empty()) b->dtype = s->dtype;

@@ -207,9 +207,67 @@ def test_narrow_and_odd_rows_at_every_cells_per_lane(nx):
     ref = oracle_solver(O, d, txs, cpml, mask, "f64", field=f0).run()
     want = [ref.field, ref.tx["src"], ref.tx["vn_rec"]] + list(ref.tx["rec"]) + list(ref.mp) + list(ref.mv)
     for cpl in (0, 1, 2, 4):
+        c = cpl or (4 if nx > 128 else (2 if nx > 64 else 1))
+        if not x_slabs_apart(nx, p, c):
+            # a lane would hold cells of both x slabs (nx = 5 at 4 cells per
+            # lane): the fused path refuses, the automatic one runs two-pass
+            with pytest.raises(L.AdfError, match="fused path"):
+                gpu_run(d, txs, cpml, mask, np.float64, "fast", field=f0, path="fused", cells_per_lane=cpl)
+            f, g, cp = gpu_run(d, txs, cpml, mask, np.float64, "fast", field=f0, cells_per_lane=cpl)
+            got = [f, g.src, g.vn_rec] + list(g.rec) + list(cp.p) + list(cp.v)
+            for q, (a, b) in enumerate(zip(got, want)):
+                assert np.array_equal(a, b), (cpl, "auto", q)
+            continue
         for pf in (1, 0):
             f, g, cp = gpu_run(d, txs, cpml, mask, np.float64, "fast", field=f0, path="fused", block_z=2,
                                pipeline=pf, cells_per_lane=cpl)
             got = [f, g.src, g.vn_rec] + list(g.rec) + list(cp.p) + list(cp.v)
             for q, (a, b) in enumerate(zip(got, want)):
                 assert np.array_equal(a, b), (cpl, pf, q)
+
+
+def x_slabs_apart(nx, p, cpl):
+    """No lane of cpl cells (starting at multiples of cpl) holds updated
+    cells of both x CPML slabs of a phase (pressure: cells [0, nx-1), slab
+    width p; velocity: cells [1, nx), width p+1)."""
+    for w, lo, hi in ((p, 0, nx - 1), (p + 1, 1, nx)):
+        if w <= 0:
+            continue
+        for i0 in range(0, nx, cpl):
+            cells = [i for i in range(i0, min(i0 + cpl, nx)) if lo <= i < hi]
+            if any(i < w for i in cells) and any(w <= i and i >= nx - w for i in cells):
+                return False
+    return True
+
+
+@pytest.mark.parametrize("p", [63, 64])
+def test_x_slabs_that_meet_take_the_two_pass_kernels(p):
+    """At nx = 130 (4 cells per lane) a 64-cell CPML puts the velocity
+    phase's two x slabs side by side (the library allows npml up to
+    (min(nx, ny, nz) - 1) / 2), and the lane holding cells 64-67 has
+    cells of both (64 on the left, 65-67 on the right, where the memory
+    coefficients are not zero).  The fused routines address a lane's x-slab
+    memory from one side, so that grid runs on the two-pass kernels (slab
+    test per cell) and refuses the fused path; with a 63-cell CPML the slabs
+    stay apart and the grid runs fused.  Float64 equals the oracle bit for
+    bit either way, ten steps from a random field (the memory of those
+    cells turns nonzero from the first step)."""
+    nx = 130
+    assert x_slabs_apart(nx, p, 4) == (p == 63)
+    d, txs, cpml, mask = build(n=(nx, 129, 129), nt=10, npml=p, n_rec=0, src_cells=(0.3, 0.2, -0.1),
+                               src_dir=(1.0, 0.3, -0.2, 0.1))
+    rng = np.random.default_rng(p)
+    f0 = rng.standard_normal((4, d.nz, d.ny, d.nx))
+    ref = oracle_solver(O, d, txs, cpml, mask, "f64", field=f0).run()
+    want = [ref.field, ref.tx["src"], ref.tx["vn_rec"]] + list(ref.tx["rec"]) + list(ref.mp) + list(ref.mv)
+    runs = [dict()]
+    if p == 63:
+        runs.append(dict(path="fused", pipeline=1, block_z=3))
+    else:
+        with pytest.raises(L.AdfError, match="fused path"):
+            gpu_run(d, txs, cpml, mask, np.float64, "fast", field=f0, path="fused")
+    for kw in runs:
+        f, g, cp = gpu_run(d, txs, cpml, mask, np.float64, "fast", field=f0, **kw)
+        got = [f, g.src, g.vn_rec] + list(g.rec) + list(cp.p) + list(cp.v)
+        for q, (a, b) in enumerate(zip(got, want)):
+            assert np.array_equal(a, b), (kw, q)
