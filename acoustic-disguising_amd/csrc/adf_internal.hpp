@@ -117,7 +117,7 @@ struct adf_solver {
     int waves = 0;     // waves per workgroup of the fused kernel (0 auto)
     int block_z_gen = 0;  // planes per general tile (0 auto)
     int pipeline = 1;     // pipelined (prefetching) tile routines
-    int store_nt = 0;     // non-temporal stores in the pipelined routines
+    int store_nt = -1;    // non-temporal field stores in the pipelined routines: 1, 0, -1 automatic
     int launch = 0;       // ADF_SWEEP_* (one launch, two streams, two serial)
     int tx_launch = 1;    // ADF_OPT_TX_LAUNCH
     int cpl_opt = 0;      // ADF_OPT_CELLS_PER_LANE (0 auto)

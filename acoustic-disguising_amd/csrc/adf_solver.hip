@@ -80,6 +80,8 @@ void device_pool_trim(int dev, size_t keep) {
 }
 
 constexpr size_t FIELD_SLACK = 256;
+// MI355X's last-level (Infinity) cache
+constexpr int64_t LLC_BYTES = int64_t(256) << 20;
 
 static int dmalloc(void** p, size_t bytes, const char* what) {
     *p = nullptr;
@@ -845,7 +847,7 @@ int solver_set_option(adf_solver* s, int option, int64_t value) {
         s->waves = (int)value;
         touch(s);
     } else if (option == ADF_OPT_STORE_NT) {
-        if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_STORE_NT must be 0 or 1");
+        if (value < -1 || value > 1) return fail(ADF_EINVAL, "ADF_OPT_STORE_NT must be -1, 0 or 1");
         s->store_nt = (int)value;
     } else if (option == ADF_OPT_PIPELINE) {
         if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_PIPELINE must be 0 or 1");
@@ -1555,6 +1557,13 @@ static int batch_build(adf_batch* b) {
     std::vector<TxDev<T>> td(n);
     std::vector<int> it0(n), cur(n);
     std::map<std::tuple<int, int, int, int, int, int, int>, std::array<std::vector<uint2>, TILE_LISTS>> glists;
+    // members left on the automatic store hint (ADF_OPT_STORE_NT = -1)
+    // store non-temporally when the batch's field buffers exceed the GPU's
+    // last-level cache, where ordinary stores would only evict the next
+    // step's reads
+    int64_t field_bytes = 0;
+    for (adf_solver* o : b->members) field_bytes += 8 * o->n3 * (int64_t)o->es;
+    const bool auto_nt = field_bytes > LLC_BYTES;
     // total work decides the z-block: enough tiles for every CU
     int64_t cells = 0;
     for (adf_solver* s : b->members) cells += s->n3;
@@ -1600,7 +1609,8 @@ static int batch_build(adf_batch* b) {
         const std::tuple<int, int, int, int, int, int, int> key(cpl, ty, s->waves ? s->waves : 1, s->pipeline,
                                                                 s->launch == ADF_SWEEP_ONE_LAUNCH ? 1 : 0,
                                                                 s->dom.nx % cpl != 0 ? 1 : 0,
-                                                                s->pipeline && s->store_nt ? 1 : 0);
+                                                                s->pipeline && (s->store_nt > 0 ||
+                                                                                (s->store_nt < 0 && auto_nt)) ? 1 : 0);
         auto& G = glists[key];
         for (int c : {LIST_INT, LIST_SIDE}) G[c].insert(G[c].end(), lists[c].begin(), lists[c].end());
     }

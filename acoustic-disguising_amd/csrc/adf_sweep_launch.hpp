@@ -23,7 +23,7 @@ static void launch_list(adf_solver* s, const Fused<T>& F, hipStream_t st) {
     const int W = s->waves ? s->waves : 1;
     const dim3 g((unsigned)((n + W - 1) / W));
     const uint2* tl = (const uint2*)s->tiles[LIST];
-    if (s->pipeline && s->store_nt) {
+    if (s->pipeline && s->store_nt > 0) {  // automatic: ordinary stores for a single solve
         hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, ST_NT, RAG>), g, dim3(64 * W), 0, st, F, tl, n);
         return;
     }

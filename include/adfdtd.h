@@ -225,7 +225,10 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
  *                        field rows with the non-temporal hint (faster
  *                        where the two field buffers exceed the GPU's
  *                        last-level cache, e.g. 256^3; slower where they
- *                        fit); 0 (default): ordinary stores.  A batch
+ *                        fit); 0: ordinary stores; -1 (default):
+ *                        automatic, i.e. ordinary stores for a solve run
+ *                        alone, and in a batch the hint when the members'
+ *                        field buffers together exceed 256 MB.  A batch
  *                        takes each member's setting.  Results are
  *                        identical. */
 #define ADF_OPT_STORE_NT 9
