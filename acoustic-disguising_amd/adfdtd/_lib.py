@@ -41,6 +41,7 @@ ADF_OPT_PIPELINE, ADF_OPT_SWEEP_LAUNCH, ADF_OPT_STORE_NT, ADF_OPT_TX_LAUNCH, ADF
 ADF_SWEEP_ONE_LAUNCH, ADF_SWEEP_TWO_STREAMS, ADF_SWEEP_TWO_LAUNCHES = 0, 1, 2
 ADF_BATCH_PERSISTENT, ADF_BATCH_GRAPH_STEPS = 1, 2
 ADF_BW_COPY_BUF, ADF_BW_COPY_GLOBAL, ADF_BW_READ, ADF_BW_WRITE, ADF_BW_COPY_NT = 0, 1, 2, 3, 4
+ADF_BW_ZMARCH, ADF_BW_ZMARCH_NT = 5, 6
 
 
 class AdfDomain(C.Structure):

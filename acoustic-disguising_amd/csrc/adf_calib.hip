@@ -12,9 +12,18 @@
 //                      workgroup)
 //   ADF_BW_WRITE       write only (128-bit buffer stores)
 //   ADF_BW_COPY_NT     as ADF_BW_COPY_BUF with non-temporal stores
+//   ADF_BW_ZMARCH      the fused sweep's access pattern without its
+//                      arithmetic, CPML or mask: each buffer holds four
+//                      channels of 256-cell rows (256 rows per plane); a
+//                      wave owns 2 rows, marches 4 planes in z and per
+//                      plane loads rows -1..1 of three channels and -1..2
+//                      of the fourth (the next plane's third channel) one
+//                      plane ahead, then stores its 2 rows of all four;
+//                      launches alternate between the two buffers
+//   ADF_BW_ZMARCH_NT   as ADF_BW_ZMARCH with non-temporal stores
 //
-// Each thread moves U 16-byte chunks per launch; U chunks are loaded
-// before any is stored.
+// The copy, read and write probes move U 16-byte chunks per thread per
+// launch; U chunks are loaded before any is stored.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -80,6 +89,73 @@ __global__ __launch_bounds__(BW_THREADS) void k_bw_write(void* b, uint32_t bytes
     for (int u = 0; u < BW_U; ++u) store_b128<0>(v, dst, base + u * BW_THREADS * 16, 0);
 }
 
+// One plane of a z-march tile: rows -1..1 of P, Vx and the next plane's
+// Vz, rows -1..2 of Vy (16-byte chunks, 4 cells per lane).
+struct ZPlane {
+    u32x4 p[3], vx[3], vy[4], vz[3];
+};
+
+__device__ __forceinline__ u32x4 zsum(u32x4 a, u32x4 b) { return a + b; }
+
+template <int POL>
+__global__ __launch_bounds__(64) void k_bw_zmarch(const void* a, void* b, uint32_t chb, int ny, int nz) {
+    constexpr int TY = 2, KB = 4, ROW = 1024;
+    const int ntj = ny / TY, nt = ntj * (nz / KB);
+    const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    if (t >= nt) return;
+    const int j0 = (t % ntj) * TY, k0 = (t / ntj) * KB, k1 = k0 + KB;
+    const Rsrc src = rsrc_of(a, 4u * chb), dst = rsrc_of(b, 4u * chb);
+    const int vl = (int)(threadIdx.x & 63) * 16;
+    auto rowb = [&](int j, int k) {
+        j = j < 0 ? 0 : (j >= ny ? ny - 1 : j);
+        k = k >= nz ? nz - 1 : k;
+        return (int)((uint32_t)(j + ny * k) * (uint32_t)ROW);
+    };
+    auto ch = [&](int c, int rb) { return (int)((uint32_t)rb + (uint32_t)c * chb); };
+    auto load = [&](ZPlane& L, int k) {
+#pragma unroll
+        for (int r = 0; r <= TY + 1; ++r) {
+            const int rb = rowb(j0 + r - 1, k);
+            L.vy[r] = __builtin_amdgcn_raw_buffer_load_b128(src, vl, ch(2, rb), 0);
+            if (r > TY) continue;
+            L.p[r] = __builtin_amdgcn_raw_buffer_load_b128(src, vl, rb, 0);
+            L.vx[r] = __builtin_amdgcn_raw_buffer_load_b128(src, vl, ch(1, rb), 0);
+            L.vz[r] = __builtin_amdgcn_raw_buffer_load_b128(src, vl, ch(3, rowb(j0 + r - 1, k + 1)), 0);
+        }
+    };
+    u32x4 carry[TY + 1] = {};
+    // every loaded word feeds a stored one (integer sums: no arithmetic to
+    // speak of), so no load is dead
+    auto step = [&](ZPlane& L, int k) {
+        u32x4 p1[TY + 1];
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) p1[r] = zsum(zsum(L.p[r], L.vx[r]), zsum(L.vy[r + 1] - L.vy[r], L.vz[r] - carry[r]));
+#pragma unroll
+        for (int r = 1; r <= TY; ++r) {
+            const int rb = rowb(j0 + r - 1, k);
+            store_b128<POL>(p1[r], dst, vl, rb);
+            store_b128<POL>(L.vx[r] - p1[r], dst, vl, ch(1, rb));
+            store_b128<POL>(L.vy[r] - (p1[r] - p1[r - 1]), dst, vl, ch(2, rb));
+            store_b128<POL>(L.vz[r] - p1[r], dst, vl, ch(3, rb));
+        }
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) carry[r] = L.vz[r];
+    };
+    ZPlane B0, B1;
+    load(B0, k0);
+    int k = k0;
+    for (;;) {
+        if (k + 1 < k1) load(B1, k + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        step(B0, k);
+        if (++k >= k1) break;
+        if (k + 1 < k1) load(B0, k + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        step(B1, k);
+        if (++k >= k1) break;
+    }
+}
+
 struct DevMem {
     void* p = nullptr;
     ~DevMem() {
@@ -95,8 +171,13 @@ extern "C" int adf_bandwidth_probe(adf_ctx* ctx, int kind, int64_t bytes, int re
     if (bytes < chunk || bytes % chunk || bytes > ((int64_t)1 << 30))
         return fail(ADF_EINVAL, "adf_bandwidth_probe: bytes must be a multiple of " + std::to_string(chunk) +
                                     " and at most 1 GiB");
-    if (reps < 1 || kind < ADF_BW_COPY_BUF || kind > ADF_BW_COPY_NT)
+    if (reps < 1 || kind < ADF_BW_COPY_BUF || kind > ADF_BW_ZMARCH_NT)
         return fail(ADF_EINVAL, "adf_bandwidth_probe: bad kind or reps");
+    const bool zm = kind == ADF_BW_ZMARCH || kind == ADF_BW_ZMARCH_NT;
+    // z-march: four channels of 256 rows x nz planes of 1 KB rows, nz a
+    // multiple of the 4-plane tiles
+    if (zm && bytes % ((int64_t)4 * 1024 * 256 * 4))
+        return fail(ADF_EINVAL, "adf_bandwidth_probe: z-march probes take a multiple of 4 MiB");
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     DevMem a, b, sink;
@@ -107,7 +188,21 @@ extern "C" int adf_bandwidth_probe(adf_ctx* ctx, int kind, int64_t bytes, int re
     HIPCHK(hipMemsetAsync(b.p, 0, (size_t)bytes, st));
     const uint32_t nb = (uint32_t)bytes;
     const unsigned grid = (unsigned)(bytes / chunk);
+    const uint32_t zchb = (uint32_t)(bytes / 4);          // bytes per channel
+    const int zny = 256, znz = (int)(zchb / 1024 / 256);  // 256 rows of 1 KB per plane
+    const unsigned zgrid = (unsigned)((zny / 2) * (znz / 4));
+    int flip = 0;
     auto launch = [&]() {
+        if (zm) {  // alternate between the buffers: each launch reads what the last one wrote
+            const void* from = flip ? b.p : a.p;
+            void* to = flip ? a.p : b.p;
+            flip ^= 1;
+            if (kind == ADF_BW_ZMARCH)
+                hipLaunchKernelGGL(k_bw_zmarch<0>, dim3(zgrid), dim3(64), 0, st, from, to, zchb, zny, znz);
+            else
+                hipLaunchKernelGGL(k_bw_zmarch<ST_NT>, dim3(zgrid), dim3(64), 0, st, from, to, zchb, zny, znz);
+            return;
+        }
         switch (kind) {
             case ADF_BW_COPY_BUF:
                 hipLaunchKernelGGL(k_bw_copy_buf<0>, dim3(grid), dim3(BW_THREADS), 0, st, a.p, b.p, nb);

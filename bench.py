@@ -23,7 +23,7 @@ Modes for profiling (see profiles/):
   --profile-grid N    run the production-grid leg at N^3 only (301, 501 or 545;
                       --profile-steps K steps, default 50)
   --calibrate-copy    HBM ceilings of the library's streaming probe kernels
-                      (copy/read/write; --probe-kind K for one kind, the
+                      (copy/read/write/z-march; --probe-kind K for one kind, the
                       run the PMC scale factors are read from)
 """
 from __future__ import annotations
@@ -337,22 +337,26 @@ def calibrate_copy(kinds=None, sizes=(1 << 28, 1 << 30), reps=20):
     (adf_bandwidth_probe): copy with the sweep's 128-bit buffer loads and
     stores, copy with global loads and stores, read only, write only, copy
     with non-temporal stores; buffers of 256 MiB (the size of the GPU's
-    last-level cache) and 1 GiB."""
+    last-level cache) and 1 GiB.  The z-march probes (the sweep's access
+    pattern without its arithmetic, ordinary and non-temporal stores) also
+    run at 64 MiB per buffer, where both buffers fit the last-level cache."""
     import ctypes as C
 
     from adfdtd import _lib as L
     from adfdtd.solver import context
 
     names = {L.ADF_BW_COPY_BUF: "copy_buffer", L.ADF_BW_COPY_GLOBAL: "copy_global", L.ADF_BW_READ: "read",
-             L.ADF_BW_WRITE: "write", L.ADF_BW_COPY_NT: "copy_nontemporal"}
+             L.ADF_BW_WRITE: "write", L.ADF_BW_COPY_NT: "copy_nontemporal", L.ADF_BW_ZMARCH: "zmarch",
+             L.ADF_BW_ZMARCH_NT: "zmarch_nontemporal"}
+    zmarch = (L.ADF_BW_ZMARCH, L.ADF_BW_ZMARCH_NT)
     lib = L.load()
     ctx = context(0)
     out = {}
     for kind in kinds if kinds is not None else sorted(names):
-        for nb in sizes:
+        for nb in ((1 << 26,) + tuple(sizes) if kind in zmarch and kinds is None else sizes):
             ms = C.c_double()
             L.check(lib.adf_bandwidth_probe(ctx, kind, nb, reps, C.byref(ms)), "adf_bandwidth_probe")
-            moved = nb * (2 if kind in (L.ADF_BW_COPY_BUF, L.ADF_BW_COPY_GLOBAL, L.ADF_BW_COPY_NT) else 1)
+            moved = nb * (2 if kind in (L.ADF_BW_COPY_BUF, L.ADF_BW_COPY_GLOBAL, L.ADF_BW_COPY_NT) + zmarch else 1)
             out[f"{names[kind]}_{nb >> 20}MiB"] = {"ms": round(ms.value, 4), "bytes_moved": moved,
                                                    "GBps": round(moved / (ms.value * 1e-3) / 1e9, 1),
                                                    "frac_of_peak": round(moved / (ms.value * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}

@@ -382,6 +382,10 @@ int adf_kh_last_ms(double* ms);
 #define ADF_BW_READ 2        /* read only */
 #define ADF_BW_WRITE 3       /* write only */
 #define ADF_BW_COPY_NT 4     /* copy, non-temporal stores */
+#define ADF_BW_ZMARCH 5      /* the fused sweep's access pattern without its arithmetic: four channels of
+                                256-cell rows, 2-row x 4-plane tiles marched in z with halo rows, one plane
+                                prefetched; bytes (a multiple of 4 MiB) per buffer, launches alternate */
+#define ADF_BW_ZMARCH_NT 6   /* as ADF_BW_ZMARCH with non-temporal stores */
 int adf_bandwidth_probe(adf_ctx* ctx, int kind, int64_t bytes, int reps, double* ms_per_launch);
 
 /* ----------------------------------------------------------------- server
