@@ -32,7 +32,7 @@ struct TxDev {
 // other chunks' queues in turn; the last wave to finish resets the queue
 // heads for the next launch.  POL: cache policy of the pipelined routines'
 // field stores (0 or ST_NT).
-template <typename T, bool EXACT, int CPL, bool PF, bool RAG, int POL = 0>
+template <typename T, bool EXACT, int CPL, bool PF, bool RAG, int POL = 0, bool INTONLY = false>
 __global__ __launch_bounds__(256) void k_sweep_batch(const Fused<T>* __restrict__ fs, const int* __restrict__ cur,
                                                      const uint2* __restrict__ tiles, int ntiles, unsigned* ctr,
                                                      int persistent) {
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_sweep_batch(const Fused<T>* __restrict_
         const uint2 e = tiles[t];
         const int s = (int)((e.x >> 16) & 0xfffu);
         const Fused<T> F = fs[2 * s + cur[s]];
-        sweep_tile<T, EXACT, CPL, PF, POL, RAG>(F, e);
+        sweep_tile<T, EXACT, CPL, PF, POL, RAG, INTONLY>(F, e);
     }
     if (!persistent) return;
     if ((threadIdx.x & 63) == 0) {

@@ -1101,11 +1101,21 @@ static int open_rows(adf_solver* s, std::vector<uint8_t>& open) {
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     HIPCHK(hipMemcpy(bits.data(), s->mask, bits.size() * 4, hipMemcpyDeviceToHost));
     const uint32_t last = (nx % 32) ? ((1u << (nx % 32)) - 1u) : 0xffffffffu;
+    // Bits whose value no routine reads are left out: P at i = nx-1 and Vx
+    // at i = 0 are never updated (the sweep keeps their old values
+    // whatever the mask), nor are Vy on row j = 0 and Vz on plane k = 0
+    // (only general tiles hold them, and those keep the old values there).
+    // The mask rule closes Vx at i = 0 (and Vy/Vz on the first row/plane)
+    // everywhere, so without this no row of a masked grid would be open.
+    const int64_t lastw = (nx - 1) >> 5;
     for (int64_t c = 0; c < 4; ++c)
         for (int64_t r = 0; r < ny * nz; ++r) {
+            if ((c == 2 && r % ny == 0) || (c == 3 && r < ny)) continue;
             const uint32_t* w = &bits[(size_t)((c * ny * nz + r) * nwx)];
             for (int64_t q = 0; q < nwx; ++q) {
-                const uint32_t want = q == nwx - 1 ? last : 0xffffffffu;
+                uint32_t want = q == nwx - 1 ? last : 0xffffffffu;
+                if (c == 0 && q == lastw) want &= ~(1u << ((nx - 1) & 31));
+                if (c == 1 && q == 0) want &= ~1u;
                 if ((w[q] & want) != want) { open[(size_t)r] = 0; break; }
             }
         }
@@ -1213,8 +1223,13 @@ static void split_tiles(const adf_solver* s, int ty, int kb, int kbg, int W, uin
                 kind = ys ? (zl ? TILE_SYZ : TILE_SY) : (zl ? TILE_SZ : TILE_INT);
             }
             if (kind == TILE_INT) {
-                for (int g = 0; g < nseg; ++g)
-                    lists[LIST_INT].emplace_back(z.first, j0, tile(j0, ty, z.first, z.second, kind, g));
+                // the interior list holds unmasked interior tiles only (its
+                // launch may compile their routine alone); masked interior
+                // tiles go with the side tiles
+                for (int g = 0; g < nseg; ++g) {
+                    const uint2 e = tile(j0, ty, z.first, z.second, kind, g);
+                    lists[(e.x >> 31) ? LIST_SIDE : LIST_INT].emplace_back(z.first, j0, e);
+                }
                 continue;
             }
             for (int j = j0; j < j1; j += tys)

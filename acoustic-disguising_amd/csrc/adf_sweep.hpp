@@ -585,13 +585,21 @@ enum { LIST_INT = 0, LIST_SIDE = 1, LIST_ALL = 2, TILE_LISTS = 3 };
 // otherwise the plane-at-a-time routines (fused_tile for all side tiles).
 // General tiles always take fused_tile.
 // RAG: the solve's rows are not a multiple of CPL long (fused_tile).
-template <typename T, bool EXACT, int CPL, bool PF, int POL = 0, bool RAG = false>
+// INTONLY: the tile is an unmasked interior tile (the interior list holds
+// only those): a kernel instantiated with INTONLY compiles the one routine
+// they take, so its register count, and with it the waves per SIMD, is
+// that routine's alone.
+template <typename T, bool EXACT, int CPL, bool PF, int POL = 0, bool RAG = false, bool INTONLY = false>
 __device__ __forceinline__ void sweep_tile(const Fused<T>& F, uint2 e) {
     const int j0 = (int)(e.x & 0xffffu), k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0x3ffu);
     const int seg = (int)((e.y >> 10) & 0x3fu);
     const bool masked = (e.x >> 31) != 0;
     const int kind = (int)((e.x >> 28) & 7u);
     constexpr int TI = sweep_ty_int((int)sizeof(T), CPL), TS = sweep_ty_side(CPL);
+    if constexpr (INTONLY) {
+        sweep_tile_pf<T, EXACT, CPL, TI, false, false, POL, true, RAG>(F, j0, k0, k1, false, seg);
+        return;
+    }
     if (kind == TILE_INT) {
         if constexpr (PF) {
             if (masked) sweep_tile_pf<T, EXACT, CPL, TI, false, false, POL, false, RAG>(F, j0, k0, k1, true, seg);
@@ -613,13 +621,13 @@ __device__ __forceinline__ void sweep_tile(const Fused<T>& F, uint2 e) {
 // One solve's tiles of one list (entries as xcd_tile describes); W waves
 // per workgroup take W consecutive tiles.  POL: cache policy of the
 // stores (0 or ST_NT).
-template <typename T, bool EXACT, int CPL, bool PF, int POL, bool RAG>
+template <typename T, bool EXACT, int CPL, bool PF, int POL, bool RAG, bool INTONLY = false>
 __global__ __launch_bounds__(256) void k_sweep(Fused<T> F, const uint2* __restrict__ tiles, int ntiles) {
     const int W = (int)(blockDim.x >> 6);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x) * W + w;
     if (t >= ntiles) return;
-    sweep_tile<T, EXACT, CPL, PF, POL, RAG>(F, tiles[t]);
+    sweep_tile<T, EXACT, CPL, PF, POL, RAG, INTONLY>(F, tiles[t]);
 }
 
 }  // namespace adf

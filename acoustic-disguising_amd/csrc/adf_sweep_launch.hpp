@@ -23,12 +23,15 @@ static void launch_list(adf_solver* s, const Fused<T>& F, hipStream_t st) {
     const int W = s->waves ? s->waves : 1;
     const dim3 g((unsigned)((n + W - 1) / W));
     const uint2* tl = (const uint2*)s->tiles[LIST];
+    // the interior list holds only unmasked interior tiles: its launch
+    // takes the kernel that compiles their routine alone
+    constexpr bool IO = LIST == LIST_INT;
     if (s->pipeline && s->store_nt > 0) {  // automatic: ordinary stores for a single solve
-        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, ST_NT, RAG>), g, dim3(64 * W), 0, st, F, tl, n);
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, ST_NT, RAG, IO>), g, dim3(64 * W), 0, st, F, tl, n);
         return;
     }
     if (s->pipeline) {
-        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, 0, RAG>), g, dim3(64 * W), 0, st, F, tl, n);
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, 0, RAG, IO>), g, dim3(64 * W), 0, st, F, tl, n);
         return;
     }
     hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, false, 0, RAG>), g, dim3(64 * W), 0, st, F, tl, n);
@@ -75,14 +78,15 @@ static void batch_launch_list(adf_batch* b, const adf_batch::Group& g, hipStream
     const Fused<T>* fs = (const Fused<T>*)b->fs;
     const int pers = b->persistent ? 1 : 0;
     const dim3 blk(64 * g.waves);
+    constexpr bool IO = LIST == LIST_INT;  // unmasked interior tiles only
     if (g.pf && g.store_nt) {
-        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, true, RAG, ST_NT>), dim3(g.grid[LIST]), blk, 0, st, fs,
+        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, true, RAG, ST_NT, IO>), dim3(g.grid[LIST]), blk, 0, st, fs,
                            b->cur, g.tiles[LIST], g.n[LIST], g.ctr[LIST], pers);
         return;
     }
     if (g.pf) {
-        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, true, RAG>), dim3(g.grid[LIST]), blk, 0, st, fs, b->cur,
-                           g.tiles[LIST], g.n[LIST], g.ctr[LIST], pers);
+        hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, true, RAG, 0, IO>), dim3(g.grid[LIST]), blk, 0, st, fs,
+                           b->cur, g.tiles[LIST], g.n[LIST], g.ctr[LIST], pers);
         return;
     }
     hipLaunchKernelGGL((k_sweep_batch<T, EXACT, CPL, false, RAG>), dim3(g.grid[LIST]), blk, 0, st, fs, b->cur,
