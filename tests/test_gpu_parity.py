@@ -105,6 +105,31 @@ def test_random_field_every_cell_active(kind, shape, size, path):
         assert all(v == 0.0 for v in errs.values()), errs
 
 
+@pytest.mark.parametrize("kind", ["f64", "fast"])
+def test_mask_closed_where_no_routine_reads_it(kind, path):
+    """A mask closed only where no update reads it (P at i = nx-1, Vx at
+    i = 0, Vy on row 0, Vz on plane 0) leaves every row open for the
+    tile split, so those tiles run the unmasked routines; a few rows also
+    have cells closed for real (closed pressure cells and their faces).
+    Float64 equals the oracle with the same mask bit for bit, from a random
+    field."""
+    d, txs, cpml, _ = build(n=(40, 29, 27), nt=10, npml=5, n_rec=10, rec_radius_cells=6)
+    m = np.ones((4, d.nz, d.ny, d.nx), np.uint8)
+    m[0, :, :, d.nx - 1] = 0
+    m[1, :, :, 0] = 0
+    m[2, :, 0, :] = 0
+    m[3, 0, :, :] = 0
+    rng = np.random.default_rng(11)
+    for _ in range(6):  # closed cells for real, away from the CPML: the tiles around them stay masked
+        i, j, k = rng.integers(8, d.nx - 8), rng.integers(8, d.ny - 8), rng.integers(8, d.nz - 8)
+        m[:, k, j, i] = 0
+        m[1, k, j, i + 1] = m[2, k, j + 1, i] = m[3, k + 1, j, i] = 0
+    f0 = np.random.default_rng(5).standard_normal((4, d.nz, d.ny, d.nx))
+    errs = _compare(d, txs, cpml, m, kind, field=f0 if kind == "f64" else f0.astype(np.float32), path=path)
+    if kind == "f64":
+        assert all(v == 0.0 for v in errs.values()), errs
+
+
 @pytest.mark.parametrize("kind", ["f64", "ref", "fast"])
 def test_anisotropic_spacing(kind, path):
     # dx != dy != dz (different xmax, ymax, zmax per axis) and nx != ny != nz:
