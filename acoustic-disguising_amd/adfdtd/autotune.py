@@ -76,9 +76,12 @@ def _scratch(s: Solver) -> Solver:
 
 
 def _apply(s: Solver, shape: dict) -> None:
+    """Set a shape's options on s; a shape without a store hint leaves the
+    automatic one (ADF_OPT_STORE_NT = -1), so candidates are timed with the
+    stores they would run with."""
     s.set_path("auto", shape["tile_y"], shape["block_z"], shape["waves"], shape["block_z_gen"],
                pipeline=shape.get("pipeline", 1), launch=shape.get("launch", L.ADF_SWEEP_ONE_LAUNCH),
-               store_nt=shape.get("store_nt", 0), cells_per_lane=shape.get("cpl", 0))
+               store_nt=shape.get("store_nt", -1), cells_per_lane=shape.get("cpl", 0))
 
 
 def tune_solver(s: Solver, steps: int = 20, candidates: Optional[Sequence[dict]] = None,
@@ -135,7 +138,7 @@ def tune_solver(s: Solver, steps: int = 20, candidates: Optional[Sequence[dict]]
     return best
 
 
-def tune_batch(members: Sequence[Solver], steps: int = 24, candidates: Optional[Sequence[dict]] = None,
+def tune_batch(members: Sequence[Solver], steps: int = 32, candidates: Optional[Sequence[dict]] = None,
                path: Optional[str] = None, use_cache: bool = True, batch=None) -> dict:
     """Pick the tiling of a batch of these solvers (by the device time per
     batched step on scratch copies) and apply it to every member (and to
