@@ -1103,10 +1103,11 @@ static int open_rows(adf_solver* s, std::vector<uint8_t>& open) {
     const uint32_t last = (nx % 32) ? ((1u << (nx % 32)) - 1u) : 0xffffffffu;
     // Bits whose value no routine reads are left out: P at i = nx-1 and Vx
     // at i = 0 are never updated (the sweep keeps their old values
-    // whatever the mask), nor are Vy on row j = 0 and Vz on plane k = 0
-    // (only general tiles hold them, and those keep the old values there).
-    // The mask rule closes Vx at i = 0 (and Vy/Vz on the first row/plane)
-    // everywhere, so without this no row of a masked grid would be open.
+    // whatever the mask); Vy on row j = 0 and Vz on plane k = 0 only take
+    // the mirrors of row / plane 1 (general tiles hold them, and their
+    // update keeps the old values there).  The mask rule closes Vx at
+    // i = 0 (and Vy/Vz on the first row/plane) everywhere, so without this
+    // no row of a masked grid would be open.
     const int64_t lastw = (nx - 1) >> 5;
     for (int64_t c = 0; c < 4; ++c)
         for (int64_t r = 0; r < ny * nz; ++r) {

@@ -329,8 +329,8 @@ __device__ __forceinline__ void sweep_tile_pf(const Fused<T>& F, int j0, int k0,
     // whose positions all lie in the slab row's extent stores one vector
     // (its cells outside the update range carry their memory unchanged, see
     // xslab).  A lane reaching past the row's extent (into the neighbouring
-    // row's memory) stores its updated cells one by one.  The other lanes' offsets are out of range (those stores
-    // are dropped).
+    // row's memory) stores its updated cells one by one.  The other lanes'
+    // offsets are out of range (those stores are dropped).
     const bool stP = XL.own && cpP && XP.in, stV = XL.own && cpV && XV.in;
     const int sPv = (stP && XP.ext == FULL) ? vP : (int)OOR;
     const int sVv = (stV && XV.ext == FULL) ? vV : (int)OOR;
