@@ -95,17 +95,41 @@ __device__ __forceinline__ int count_below(const double* g, int n, double v, boo
     return lo;
 }
 
-// cum[r][u] = sum_{t'=1}^{u} in[t', r], u = 0..nt (double)
+// cum[r][u] = sum_{t'=1}^{u} in[t', r], u = 0..nt (double).  A workgroup
+// takes PX_ROWS surface points; their inputs pass through LDS in chunks of
+// PX_CH samples (coalesced loads and stores), and one thread per point adds
+// its samples in time order, the order of a serial loop.
+constexpr int PX_ROWS = 16, PX_CH = 256;
+
 template <typename T>
-__global__ void k_kh_prefix(const T* in, int nt, int no, double* cum) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= no) return;
+__global__ __launch_bounds__(64) void k_kh_prefix(const T* in, int nt, int no, double* cum) {
+    __shared__ T tile[PX_ROWS][PX_CH + 1];
+    __shared__ double ct[PX_ROWS][PX_CH + 1];
+    const int lane = (int)threadIdx.x;
+    const int r0 = (int)blockIdx.x * PX_ROWS;
+    const int nr = min(PX_ROWS, no - r0);
     double acc = 0.0;
-    double* c = cum + (size_t)r * (nt + 1);
-    c[0] = 0.0;
-    for (int u = 1; u <= nt; ++u) {
-        acc += (double)in[(size_t)r * nt + (u - 1)];
-        c[u] = acc;
+    if (lane < nr) cum[(size_t)(r0 + lane) * (nt + 1)] = 0.0;
+    for (int u0 = 0; u0 < nt; u0 += PX_CH) {
+        const int nu = min(PX_CH, nt - u0);
+        __syncthreads();  // the last chunk's ct is stored
+        for (int q = 0; q < nr; ++q)
+#pragma unroll
+            for (int c = 0; c < PX_CH; c += 64)
+                tile[q][c + lane] = c + lane < nu ? in[(size_t)(r0 + q) * nt + u0 + c + lane] : (T)0;
+        __syncthreads();
+        if (lane < nr) {
+#pragma unroll 8
+            for (int m = 0; m < PX_CH; ++m) {
+                acc += (double)tile[lane][m];
+                ct[lane][m] = acc;
+            }
+        }
+        __syncthreads();
+        for (int q = 0; q < nr; ++q)
+#pragma unroll
+            for (int c = 0; c < PX_CH; c += 64)
+                if (c + lane < nu) cum[(size_t)(r0 + q) * (nt + 1) + 1 + u0 + c + lane] = ct[q][c + lane];
     }
 }
 
@@ -119,9 +143,11 @@ __global__ void k_kh_prefix(const T* in, int nt, int no, double* cum) {
 // lane needs for R taps lie in two R-long segments of Is: the segments
 // alternate as the taps advance, so each group of R taps loads one new
 // segment (R LDS reads for R^2 multiply-adds) and its R kernel samples as
-// 16-byte broadcasts.  R is odd, so the 64 lanes'
-// segments start in distinct LDS banks.  R = 17 for float and double: the
-// register budget of two waves per SIMD.
+// 16-byte broadcasts.  The 64 lanes' segments start in distinct LDS banks:
+// R = 17 for double (odd), R = 16 for float with one unused word after
+// every 16 input samples.  Float: two blocks of 1024 outputs cover 2000
+// steps (1088-output blocks computed 176 outputs past the end), and the
+// p_p/p_v/v_p kernels fit three waves per SIMD; double: two waves.
 //
 // T = float: kernel samples rounded to Float32 (eval_one_kernel(; T =
 // Float32)), inputs Float32, products accumulated with Float32 FMAs over
@@ -130,16 +156,21 @@ __global__ void k_kh_prefix(const T* in, int nt, int no, double* cum) {
 // chunks of r are summed in chunk order afterwards (k_kh_reduce), so the
 // result does not depend on scheduling.
 #ifndef ADF_KH_R32
-#define ADF_KH_R32 17  // outputs per lane for float (odd); 17 keeps two waves per SIMD
+#define ADF_KH_R32 16  // outputs per lane for float: two blocks of 1024 cover 2000 steps with 2 % to spare
 #endif
 template <typename T>
 struct KC {
     static constexpr int R = sizeof(T) == 4 ? ADF_KH_R32 : 17;
     static constexpr int OUT = 64 * R;
-    static constexpr int W = 16 * R;  // taps per LDS chunk (a multiple of 2 R): 272, one window of 20 sigma / dt
+    static constexpr int W = 16 * R;  // taps per LDS chunk (a multiple of R): 256 or 272, one window of 20 sigma / dt
     // kernel samples are stored in groups of R taps at a stride of KS floats
     // (16-byte aligned groups: broadcast reads of four taps at once)
     static constexpr int KS = (R + 3) / 4 * 4;
+    // input samples: with an even R, one unused word after every R (lane l
+    // reads from word l (R + 1) + m, an odd stride: no LDS bank conflicts)
+    static constexpr int PAD = R % 2 == 0 ? 1 : 0;
+    static constexpr int IS = (OUT + W) / R * (R + PAD);
+    static __device__ __forceinline__ int at(int i) { return PAD ? i + i / R : i; }
 };
 
 template <typename T, int R>
@@ -180,7 +211,7 @@ __global__ __launch_bounds__(64) void k_kh_conv(const double* __restrict__ op, c
                                                 const double* __restrict__ cum, int rpc, double* __restrict__ dst) {
     constexpr int R = KC<T>::R, OUT = KC<T>::OUT, KW = KC<T>::W, KS = KC<T>::KS;
     __shared__ __attribute__((aligned(16))) T Kw[KW / R * KS];
-    __shared__ T Is[OUT + KW];
+    __shared__ T Is[KC<T>::IS];
     const int lane = (int)threadIdx.x;
     const int s = blockIdx.y;
     const int T0 = blockIdx.z * OUT + 1;  // first output time of the wave (1-based)
@@ -202,7 +233,7 @@ __global__ __launch_bounds__(64) void k_kh_conv(const double* __restrict__ op, c
         for (int c1 = b; c1 >= a; c1 -= KW) {
             const int cl = max(a, c1 - KW + 1);
             const int w = c1 - cl + 1;
-            const int wp = (w + 2 * R - 1) / (2 * R) * (2 * R);  // taps rounded to whole pairs of groups
+            const int ng = (w + R - 1) / R, wp = ng * R;  // taps rounded to whole groups
             __syncthreads();
             for (int kk = lane; kk < wp; kk += 64) {
                 T kv = (T)0;
@@ -216,20 +247,23 @@ __global__ __launch_bounds__(64) void k_kh_conv(const double* __restrict__ op, c
             const int ub = T0 - c1 + 1;  // u of Is[0]
             for (int i = lane; i < OUT + wp; i += 64) {
                 const int u = ub + i;
-                Is[i] = (u >= 1 && u <= nt) ? inr[u - 1] : (T)0;
+                Is[KC<T>::at(i)] = (u >= 1 && u <= nt) ? inr[u - 1] : (T)0;
             }
             __syncthreads();
-            // group g reads segments g and g+1 (Is[base + gR .. base + gR +
-            // 2R - 1]); the last segment read, wp / R, ends at OUT + wp - 1
+            // group g reads segments g and g+1 (input samples base + gR ..
+            // base + gR + 2R - 1); the last segment read, ng, ends at
+            // OUT + wp - 1.  Segment n of lane l starts at word at(R (l + n)).
             T A[R], B[R];
             const int base = R * lane;
-            kc_seg<T, R>(Is, base, A);
-            kc_seg<T, R>(Is, base + R, B);
-            for (int g = 0; g < wp / R; g += 2) {
+            auto seg = [&](int n, T(&x)[R]) { kc_seg<T, R>(Is, KC<T>::at(base + n * R), x); };
+            seg(0, A);
+            seg(1, B);
+            for (int g = 0; g < ng; g += 2) {
                 kc_block<T, R>(Kw + g * KS, A, B, acc);
-                kc_seg<T, R>(Is, base + (g + 2) * R, A);
+                if (g + 1 >= ng) break;
+                seg(g + 2, A);
                 kc_block<T, R>(Kw + (g + 1) * KS, B, A, acc);
-                if ((g + 2) * R < wp) kc_seg<T, R>(Is, base + (g + 3) * R, B);
+                if (g + 2 < ng) seg(g + 3, B);
             }
         }
         const double* cr_ = cum + (size_t)r * (nt + 1);
@@ -378,13 +412,13 @@ extern "C" int adf_kh_analytic_accumulate(adf_ctx* ctx, int which, int dtype, in
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, st));
     if (f64) {
-        hipLaunchKernelGGL((k_kh_prefix<double>), dim3((no + 255) / 256), dim3(256), 0, st, (const double*)din.p, nti,
+        hipLaunchKernelGGL((k_kh_prefix<double>), dim3((no + PX_ROWS - 1) / PX_ROWS), dim3(64), 0, st, (const double*)din.p, nti,
                            no, (double*)dcum.p);
         launch_which<double>(which, pedestal != 0, g, st, (const double*)dop.p, (const double*)don.p, no,
                              (const double*)dip.p, (const double*)dinn.p, ni, (const double*)dtg.p, nk, sigma_t, c0,
                              rho0, alpha, (const double*)din.p, nti, (const double*)dcum.p, rpc, dst);
     } else {
-        hipLaunchKernelGGL((k_kh_prefix<float>), dim3((no + 255) / 256), dim3(256), 0, st, (const float*)din.p, nti,
+        hipLaunchKernelGGL((k_kh_prefix<float>), dim3((no + PX_ROWS - 1) / PX_ROWS), dim3(64), 0, st, (const float*)din.p, nti,
                            no, (double*)dcum.p);
         launch_which<float>(which, pedestal != 0, g, st, (const double*)dop.p, (const double*)don.p, no,
                             (const double*)dip.p, (const double*)dinn.p, ni, (const double*)dtg.p, nk, sigma_t, c0,
