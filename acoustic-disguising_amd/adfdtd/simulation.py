@@ -375,7 +375,7 @@ def initial_field(d: Domain, fc: float, distance: float) -> np.ndarray:
 def run_fdtd(dom: Domain, field: np.ndarray, txs: TransceiverGrid, cpml: Cpml, va,
              start_step: int = 1, on_step: Optional[Callable] = None, gf_map=None,
              precision: str = "fast", solver=None, path: str = "auto", tile_y: int = 0, block_z: int = 0,
-             waves: int = 0, pipeline: int = 1, launch: int = 0, store_nt: int = -1, tx_launch: int = 1,
+             waves: int = 0, pipeline: int = 1, launch: int = 0, store_nt: int = -1, tx_launch: int = 2,
              cells_per_lane: int = 0):
     """`run_fdtd!(dom, field, txs, cpml, va; start_step, on_step, gf_map)`
     (src/simulation.jl:542-557) on the GPU.

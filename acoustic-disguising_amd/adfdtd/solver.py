@@ -245,7 +245,7 @@ class Solver:
 
     def set_path(self, path: str = "auto", tile_y: int = 0, block_z: int = 0, waves: int = 0,
                  block_z_gen: int = 0, pipeline: int = 1, launch: int = 0, store_nt: int = -1,
-                 tx_launch: int = 1, cells_per_lane: int = 0) -> None:
+                 tx_launch: int = 2, cells_per_lane: int = 0) -> None:
         self.set_option(L.ADF_OPT_PATH, {"auto": 0, "two-pass": 1, "fused": 2}[path])
         self.set_option(L.ADF_OPT_TILE_Y, tile_y)
         self.set_option(L.ADF_OPT_BLOCK_Z, block_z)

@@ -793,4 +793,170 @@ __global__ __launch_bounds__(256) void k_tx_step(Fused<T> F, Txs<T> X, TxChain C
     tx_record_entry<T>(P, Vx, Vy, Vz, F.nx, F.ny, F.nz, X, any ? (int)C.rec[g] : 0, any, lw, it0, H);
 }
 
+// Transceiver work carried by a single solve's sweep launch
+// (ADF_OPT_TX_LAUNCH = 2): one-wave blocks before and after the nbt tile
+// blocks.
+//   - Record blocks (first in the grid) take the record-only transceivers
+//     of the previous step, TXS_REC per block, from the sweep's old
+//     buffers: the state that step left, which the sweep only reads.  At
+//     the front they also stagger the tiles' start (the launch ran ~2 us
+//     shorter than with them after the tiles, and ~1 us shorter than
+//     without them, at 256^3).
+//   - Chain blocks (last in the grid) run k_tx_step's chain groups on the
+//     new buffers.  Each first waits until the tiles owning the cells its
+//     groups read or write have stored them: such a tile (TILE_SIGNAL in
+//     its entry) publishes the launch's epoch in tflag after an agent-scope
+//     release, and the block acquires it.  No other tile reads the new
+//     buffers or writes those cells.
+// Blocks are dispatched in index order, so every tile a chain block waits
+// for is running or done when it starts and waits for nothing itself; the
+// wait is bounded all the same (a wait that runs out sets *err and the
+// block skips its work, so the results are then invalid and the host
+// reports it).
+template <typename T>
+struct TxSweep {
+    Txs<T> X;
+    TxChain C;
+    const int* wait_off;  // chain block b waits for tiles wait[wait_off[b] .. wait_off[b + 1])
+    const int* wait;
+    int* tflag;           // per tile: epoch of the launch that last stored it
+    int* err;
+    int epoch;
+    int nbt;              // tile blocks (0: the launch carries no transceiver work)
+    int nrb;              // record blocks
+    int it_rec;           // 0-based step of the records
+    int it0;              // 0-based step of the chain
+};
+constexpr uint32_t TILE_SIGNAL = 1u << 27;  // tile entry bit (free in a single solve's entries)
+constexpr int TXS_SPIN = 1 << 17;           // polls of s_sleep 4 (~0.1 us each)
+
+template <typename T>
+__device__ __forceinline__ void tx_tile_signal(const TxSweep<T>& S, int t) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if ((threadIdx.x & 63) == 0) __hip_atomic_store(S.tflag + t, S.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Record blocks take TXS_REC record-only transceivers each, 32 lanes per
+// transceiver (tx_record_entry's lanes) in TXS_REC / 2 rounds: the loads
+// that do not read the field for every round first, then the corners, then
+// the sums in the reference's corner order, as tx_record_entry does.
+constexpr int TXS_REC = 8;
+
+template <typename T>
+__device__ __forceinline__ void tx_sweep_records(const Fused<T>& F, const TxSweep<T>& S, int rb) {
+    constexpr int NR = TXS_REC / 2;
+    const int lw = (int)(threadIdx.x & 63);
+    const int ch = (lw >> 3) & 3, q = lw & 7, g32 = lw & ~31, g0 = lw & ~7;
+    const int c = ch < 3 ? ch + 1 : 0;  // buffer channel: P, Vx, Vy, Vz
+    const Txs<T>& X = S.X;
+    const int ntx = X.ntx, nt = X.nt;
+    const size_t cs = (size_t)ntx * nt;
+    const T* Fc = F.f0 + (size_t)c * F.n3;
+    int is[NR];
+    bool any[NR];
+    TxLane<T> L[NR];
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+        const int g = rb * TXS_REC + 2 * u + (lw >> 5);
+        any[u] = g < S.C.n_rec;
+        is[u] = any[u] ? (int)S.C.rec[g] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+        const size_t o = (size_t)is[u] * nt + S.it_rec + cs * c;
+        L[u] = tx_lane_load<T>(X.idx + 3 * (is[u] + ntx * c), F.nx, F.ny, F.nz, X.w + 8 * (is[u] + ntx * c),
+                               X.rec + o, X.src + o, false, any[u], q);
+    }
+    T f[NR];
+#pragma unroll
+    for (int u = 0; u < NR; ++u) f[u] = L[u].ok ? Fc[L[u].c] : (T)0;
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+        // tx_lane_apply with the corner already loaded (records only)
+        const T prod = L[u].w * f[u];
+        T r = L[u].r0;
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq) r = r + __shfl(prod, g0 + qq, 64);
+        if (!L[u].ok) r = L[u].r0;
+        const size_t o = (size_t)is[u] * nt + S.it_rec;
+        if (L[u].ok && q == 0) X.rec[o + cs * c] = r;
+        const T rx = __shfl(r, g32, 64), ry = __shfl(r, g32 + 8, 64), rz = __shfl(r, g32 + 16, 64);
+        if (any[u] && lw == g32) {
+            const T* d = X.dir + 4 * is[u];
+            X.vn_rec[o] = rx * d[1] + ry * d[2] + rz * d[3];
+        }
+    }
+}
+
+template <typename T, bool EXACT>
+__device__ __forceinline__ void tx_sweep_chain(const Fused<T>& F, const TxSweep<T>& S, int cb) {
+    const int lw = (int)(threadIdx.x & 63);
+    const TxChain& C = S.C;
+    const int hv = lw < TXC_HDR ? C.hdr[(size_t)cb * TXC_HDR + lw] : 0;
+    auto h = [&](int i) { return __builtin_amdgcn_readlane(hv, i); };
+    const int w0 = S.wait_off[cb], w1 = S.wait_off[cb + 1];
+    bool ok = true;
+    for (int base = w0; base < w1 && ok; base += 64) {
+        const int t = base + lw < w1 ? S.wait[base + lw] : -1;
+        for (int n = 0;; ++n) {
+            const bool done =
+                t < 0 || __hip_atomic_load(S.tflag + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == S.epoch;
+            if (__all(done)) break;
+            if (n >= TXS_SPIN) {
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(4);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (!ok) {
+        if (lw == 0) atomicOr(S.err, 1);
+        return;
+    }
+    T* P = F.P1();
+    T* Vx = F.V1(0);
+    T* Vy = F.V1(1);
+    T* Vz = F.V1(2);
+    const GfHist H{};
+    bool pend = false;  // a phase with work ran since the last barrier
+    for (int L = 0; L < C.nlp; ++L) {
+        const int lo = h(L), hi = h(L + 1);
+        if (lo == hi) continue;
+        if (pend) __syncthreads();
+        for (int b = lo; b < hi; b += 8) {
+            const int g = b + (lw >> 3);
+            const bool live = g < hi;
+            tx_p_entry<T>(P, F.nx, F.ny, F.nz, S.X, live ? C.list[g] : 0u, live, lw, S.it0);
+        }
+        pend = true;
+    }
+    const int flo = h(2 * (TXC_MAXLEV + 1)), fhi = h(2 * (TXC_MAXLEV + 1) + 1);
+    if (flo < fhi) {
+        if (pend) __syncthreads();
+        for (int b = flo; b < fhi; b += 64)
+            if (b + lw < fhi) fixup_cell<T, EXACT>(F, C.list[b + lw]);
+        pend = true;
+    }
+    for (int L = 0; L < C.nlv; ++L) {
+        const int lo = h(TXC_MAXLEV + 1 + L), hi = h(TXC_MAXLEV + 2 + L);
+        if (lo == hi) continue;
+        if (pend) __syncthreads();
+        for (int b = lo; b < hi; b += 2) {
+            const int g = b + (lw >> 5);
+            const bool any = g < hi;
+            tx_v_entry<T>(Vx, Vy, Vz, F.nx, F.ny, F.nz, S.X, any ? C.list[g] : 0u, any, lw, S.it0, H);
+        }
+        pend = true;
+    }
+}
+
+// The records of the last step of a run (the sweep launches take each
+// step's records one step later): from the current buffers, which F holds
+// as its old ones.
+template <typename T>
+__global__ __launch_bounds__(64) void k_tx_records(Fused<T> F, TxSweep<T> S) {
+    tx_sweep_records<T>(F, S, (int)blockIdx.x);
+}
+
 }  // namespace adf

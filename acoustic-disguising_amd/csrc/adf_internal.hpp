@@ -101,6 +101,24 @@ struct adf_solver {
     int* txc_hdr = nullptr;
     int txc_nchain = 0, txc_nrec = 0, txc_rec_off = 0;
     bool txc_ok = false;  // the chain fits k_tx_step's limits
+    std::vector<uint32_t> txc_list_h;  // host copies of txc_list / txc_hdr
+    std::vector<int> txc_hdr_h;
+    // transceivers carried by the sweep launch (ADF_OPT_TX_LAUNCH = 2): the
+    // all-tiles list with the chain's tiles marked (TILE_SIGNAL) and moved to
+    // the front of their XCD chunks, per chain block the tiles it waits for
+    // (txs_wait_off: nchain + 1 offsets into txs_wait), per tile the epoch of
+    // its last store, a wait-ran-out flag.  txs_state: 1 built for the
+    // current tiles and chain, 0 stale, -1 not applicable.
+    std::vector<uint2> tiles_all_h;  // host copy of the all-tiles list
+    uint2* txs_tiles = nullptr;
+    int* txs_wait = nullptr;
+    int* txs_wait_off = nullptr;
+    int* txs_flag = nullptr;
+    int* txs_err = nullptr;
+    int* txs_err_h = nullptr;  // pinned host copy of *txs_err, read after a run
+    int txs_state = 0;
+    int txs_epoch = 0;
+    bool txs_pending = false;  // the last step's records are not taken yet
 
     // fused-sweep tile lists for tiling (tl_ty, tl_kb, tl_kbg, tl_w):
     // interior tiles, side tiles (CPML-slab and boundary) and all tiles
@@ -119,7 +137,7 @@ struct adf_solver {
     int pipeline = 1;     // pipelined (prefetching) tile routines
     int store_nt = -1;    // non-temporal field stores in the pipelined routines: 1, 0, -1 automatic
     int launch = 0;       // ADF_SWEEP_* (one launch, two streams, two serial)
-    int tx_launch = 1;    // ADF_OPT_TX_LAUNCH
+    int tx_launch = 2;    // ADF_OPT_TX_LAUNCH
     int cpl_opt = 0;      // ADF_OPT_CELLS_PER_LANE (0 auto)
 
     // GF extrapolation map

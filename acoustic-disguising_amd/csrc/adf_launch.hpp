@@ -10,9 +10,11 @@ namespace adf {
 
 // One sweep of solve s (old buffers -> new buffers as F describes), cpl
 // cells per lane, on stream st (side tiles on the context's side stream
-// when s->launch is ADF_SWEEP_TWO_STREAMS).
+// when s->launch is ADF_SWEEP_TWO_STREAMS).  X: the step's transceiver
+// work carried by the launch (one launch, one wave per workgroup), or
+// nullptr.
 template <typename T, bool EXACT>
-void launch_fused(adf_solver* s, const Fused<T>& F, int cpl, hipStream_t st);
+void launch_fused(adf_solver* s, const Fused<T>& F, int cpl, hipStream_t st, const TxSweep<T>* X);
 
 // One sweep of every member of batch b on stream st: the side tiles, then
 // the interior tiles.  One stream: a batched step is captured in a HIP

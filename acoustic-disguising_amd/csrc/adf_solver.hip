@@ -258,6 +258,8 @@ void solver_free(adf_solver* s) {
         dfree(s->psib[b]);
     }
     dfree(s->coef); dfree(s->mask); for (auto& t : s->tiles) dfree(t);
+    dfree(s->txs_tiles); dfree(s->txs_wait); dfree(s->txs_wait_off); dfree(s->txs_flag); dfree(s->txs_err);
+    if (s->txs_err_h) (void)hipHostFree(s->txs_err_h);
     if (s->ev_fork) hipEventDestroy(s->ev_fork);
     if (s->ev_join) hipEventDestroy(s->ev_join);
     for (hipEvent_t e : s->sweep_ev) hipEventDestroy(e);
@@ -384,6 +386,9 @@ static int build_tx_chain(adf_solver* s, const std::vector<uint8_t>& ip, const s
     s->txc_list = nullptr;
     s->txc_hdr = nullptr;
     s->txc_ok = false;
+    s->txc_list_h.clear();
+    s->txc_hdr_h.clear();
+    s->txs_state = 0;
     const int64_t ntx = s->ntx;
     if ((int)s->lev_p.size() - 1 > TXC_MAXLEV || (int)s->lev_v.size() - 1 > TXC_MAXLEV) return ADF_OK;
     const int nx = (int)s->dom.nx, ny = (int)s->dom.ny, nz = (int)s->dom.nz;
@@ -520,6 +525,8 @@ static int build_tx_chain(adf_solver* s, const std::vector<uint8_t>& ip, const s
         if (rc) return rc;
         HIPCHK(hipMemcpy(s->txc_hdr, hdr.data(), hdr.size() * 4, hipMemcpyHostToDevice));
     }
+    s->txc_list_h = std::move(list);
+    s->txc_hdr_h = std::move(hdr);
     s->txc_ok = true;
     return ADF_OK;
 }
@@ -859,7 +866,7 @@ int solver_set_option(adf_solver* s, int option, int64_t value) {
         s->cpl_opt = (int)value;
         touch(s);
     } else if (option == ADF_OPT_TX_LAUNCH) {
-        if (value != 0 && value != 1) return fail(ADF_EINVAL, "ADF_OPT_TX_LAUNCH must be 0 or 1");
+        if (value < 0 || value > 2) return fail(ADF_EINVAL, "ADF_OPT_TX_LAUNCH must be 0, 1 or 2");
         s->tx_launch = (int)value;
     } else if (option == ADF_OPT_SWEEP_LAUNCH) {
         if (value < ADF_SWEEP_ONE_LAUNCH || value > ADF_SWEEP_TWO_LAUNCHES)
@@ -1278,6 +1285,8 @@ static int prepare_tiles(adf_solver* s, int ty, int kb, int kbg) {
     split_tiles(s, ty, kb, kbg, W, 0, open, lists);
     for (int c = 0; c < TILE_LISTS; ++c)
         if ((rc = upload_tiles(lists[c], &s->tiles[c], &s->ntl[c]))) return rc;
+    s->tiles_all_h = lists[LIST_ALL];
+    s->txs_state = 0;
     if (!s->ev_fork) {
         HIPCHK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
@@ -1286,6 +1295,133 @@ static int prepare_tiles(adf_solver* s, int ty, int kb, int kbg) {
     s->tl_kb = kb;
     s->tl_kbg = kbg;
     s->tl_w = W;
+    return ADF_OK;
+}
+
+// The tile order and wait lists of transceivers carried by the sweep launch
+// (ADF_OPT_TX_LAUNCH = 2) for the current all-tiles list (tiles of ty
+// interior rows, one wave per workgroup) and chain.  A chain block waits
+// for the owner tiles of every cell its groups touch: the stencil corners
+// of its transceivers on all four channels, its fix-up cells with the two
+// pressure cells each reads and its mirror cell, and for a cell on row or
+// plane 0 also the owner of row or plane 1 (which writes the mirrored
+// velocities there).  Those tiles are marked TILE_SIGNAL and moved to the
+// front of their XCD's chunk (stable), so they are stored early in the
+// launch.  Leaves txs_state -1 when the lists do not have the expected
+// shape or when more than TXS_MAX_SIGNAL tiles would signal (the step then
+// keeps k_tx_step).
+static constexpr int TXS_MAX_SIGNAL = 64;
+static int build_txs(adf_solver* s, int ty) {
+    dfree(s->txs_tiles); dfree(s->txs_wait); dfree(s->txs_wait_off); dfree(s->txs_flag);
+    s->txs_tiles = nullptr;
+    s->txs_wait = s->txs_wait_off = s->txs_flag = nullptr;
+    s->txs_state = -1;
+    const std::vector<uint2>& L = s->tiles_all_h;
+    const int n = (int)L.size(), nb = s->txc_nchain;
+    if (!n || (int)s->txc_hdr_h.size() != nb * TXC_HDR) return ADF_OK;
+    const int64_t nx = s->dom.nx, ny = s->dom.ny, nz = s->dom.nz, ntx = s->ntx;
+    const int cpl = std::max(fused_cpl(s), 1);
+    const int nseg = (int)fused_nseg(nx, cpl), xseg = fused_xseg(nx, cpl);
+    const int tys = std::min(ty, sweep_ty_side(fused_cpl(s)));
+    // owner tile of each (plane, row, segment)
+    std::vector<int> own((size_t)(ny * nz * nseg), -1);
+    for (int t = 0; t < n; ++t) {
+        const uint2 e = L[t];
+        if (e.x & TILE_SIGNAL) return ADF_OK;
+        const int j0 = (int)(e.x & 0xffffu), kind = (int)((e.x >> 28) & 7u);
+        const int k0 = (int)(e.y >> 16), k1 = k0 + (int)(e.y & 0x3ffu), seg = (int)((e.y >> 10) & 0x3fu);
+        const int h = kind == TILE_INT ? ty : tys;
+        if (seg >= nseg || k1 > nz) return ADF_OK;
+        for (int k = k0; k < k1; ++k)
+            for (int j = j0; j < std::min<int64_t>(j0 + h, ny); ++j) {
+                int& o = own[((size_t)k * ny + j) * nseg + seg];
+                if (o >= 0) return ADF_OK;
+                o = t;
+            }
+    }
+    for (int o : own)
+        if (o < 0) return ADF_OK;
+    auto owner = [&](int64_t i, int64_t j, int64_t k) {
+        return own[((size_t)k * ny + j) * nseg + std::min<int64_t>(i / xseg, nseg - 1)];
+    };
+    std::vector<std::vector<int>> wl((size_t)nb);
+    std::vector<uint8_t> sig((size_t)n, 0);
+    for (int b = 0; b < nb; ++b) {
+        std::vector<int>& w = wl[b];
+        auto add = [&](int64_t c) {
+            const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+            w.push_back(owner(i, j, k));
+            if (j == 0 && ny > 1) w.push_back(owner(i, 1, k));
+            if (k == 0 && nz > 1) w.push_back(owner(i, j, 1));
+        };
+        auto add_tx = [&](int64_t is) {
+            for (int ch = 0; ch < 4; ++ch) {
+                const int32_t* id = &s->h_idx[(size_t)3 * (is + ntx * ch)];
+                const int64_t I = id[0], J = id[1], K = id[2];
+                if (!(I >= 1 && I < nx && J >= 1 && J < ny && K >= 1 && K < nz)) continue;
+                for (int q = 0; q < 8; ++q)
+                    add((I - 1 + (q & 1)) + nx * ((J - 1 + ((q >> 1) & 1)) + ny * (K - 1 + (q >> 2))));
+            }
+        };
+        const int* h = &s->txc_hdr_h[(size_t)b * TXC_HDR];
+        for (int x = h[0]; x < h[TXC_MAXLEV]; ++x) add_tx(s->txc_list_h[x] & TX_INDEX);
+        for (int x = h[TXC_MAXLEV + 1]; x < h[2 * TXC_MAXLEV + 1]; ++x) add_tx(s->txc_list_h[x] & TX_INDEX);
+        for (int x = h[2 * (TXC_MAXLEV + 1)]; x < h[2 * (TXC_MAXLEV + 1) + 1]; ++x) {
+            const uint32_t e = s->txc_list_h[x];
+            const int ax = (int)(e >> 30);
+            const int64_t c = e & 0x3fffffffu;
+            const int64_t st = ax == 0 ? 1 : (ax == 1 ? nx : nx * ny);
+            const int64_t j = (c / nx) % ny, k = c / (nx * ny);
+            add(c);
+            add(c - st);
+            if (ax == 1 && j == 1) add(c - nx);
+            if (ax == 2 && k == 1) add(c - nx * ny);
+        }
+        std::sort(w.begin(), w.end());
+        w.erase(std::unique(w.begin(), w.end()), w.end());
+        for (int t : w) sig[t] = 1;
+    }
+    // a signal tile's agent-scope release writes back its XCD's L2: worth it
+    // for the few tiles of a point source's chain, not for a surface's
+    if (std::count(sig.begin(), sig.end(), (uint8_t)1) > TXS_MAX_SIGNAL) return ADF_OK;
+    // signal tiles first within each XCD chunk (xcd_tile's chunks of n)
+    std::vector<int> pos((size_t)n);
+    std::vector<uint2> out;
+    out.reserve(n);
+    for (int x = 0; x < 8; ++x) {
+        const int a = x * (n >> 3) + std::min(x, n & 7), z = a + (n >> 3) + (x < (n & 7) ? 1 : 0);
+        for (int pass = 1; pass >= 0; --pass)
+            for (int t = a; t < z; ++t)
+                if (sig[t] == pass) {
+                    pos[t] = (int)out.size();
+                    out.push_back(pass ? make_uint2(L[t].x | TILE_SIGNAL, L[t].y) : L[t]);
+                }
+    }
+    std::vector<int> off(1, 0), all;
+    for (int b = 0; b < nb; ++b) {
+        for (int t : wl[b]) all.push_back(pos[t]);
+        off.push_back((int)all.size());
+    }
+    if (all.empty()) all.push_back(0);
+    int rc;
+    if ((rc = dmalloc((void**)&s->txs_tiles, out.size() * sizeof(uint2), "transceiver tiles")) ||
+        (rc = dmalloc((void**)&s->txs_wait_off, off.size() * 4, "transceiver tiles")) ||
+        (rc = dmalloc((void**)&s->txs_wait, all.size() * 4, "transceiver tiles")) ||
+        (rc = dmalloc((void**)&s->txs_flag, (size_t)n * 4, "transceiver tiles")))
+        return rc;
+    if (!s->txs_err) {
+        if ((rc = dmalloc((void**)&s->txs_err, 4, "transceiver tiles"))) return rc;
+        HIPCHK(hipMemset(s->txs_err, 0, 4));
+    }
+    if (!s->txs_err_h) {
+        HIPCHK(hipHostMalloc((void**)&s->txs_err_h, 4, hipHostMallocDefault));
+        *s->txs_err_h = 0;
+    }
+    HIPCHK(hipMemcpy(s->txs_tiles, out.data(), out.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->txs_wait_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->txs_wait, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(s->txs_flag, 0, (size_t)n * 4));
+    s->txs_state = 1;
     return ADF_OK;
 }
 
@@ -1303,32 +1439,64 @@ static void sweep_mark(adf_solver* s, hipStream_t st, bool sampled) {
     (void)hipEventRecord(s->sweep_ev[s->sweep_used++], st);
 }
 
-// One time step it (1-based) of forward_onestep! on stream st.
+static TxChain tx_chain_of(const adf_solver* s) {
+    TxChain C{};
+    C.list = s->txc_list;
+    C.hdr = s->txc_hdr;
+    C.rec = s->txc_list + s->txc_rec_off;
+    C.nchain = s->txc_nchain;
+    C.n_rec = s->txc_nrec;
+    C.nlp = (int)s->lev_p.size() - 1;
+    C.nlv = (int)s->lev_v.size() - 1;
+    return C;
+}
+
+template <typename T>
+static TxSweep<T> tx_sweep_of(adf_solver* s, const Txs<T>& X) {
+    TxSweep<T> S{};
+    S.X = X;
+    S.C = tx_chain_of(s);
+    S.wait_off = s->txs_wait_off;
+    S.wait = s->txs_wait;
+    S.tflag = s->txs_flag;
+    S.err = s->txs_err;
+    S.nrb = (S.C.n_rec + TXS_REC - 1) / TXS_REC;
+    return S;
+}
+
+// One time step it (1-based) of forward_onestep! on stream st.  txs: the
+// step's transceivers ride in the sweep launch (ADF_OPT_TX_LAUNCH = 2), the
+// records of the previous step with them.
 template <typename T, bool EXACT>
-static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int ty) {
+static void step_impl(adf_solver* s, int64_t it, hipStream_t st, bool fused, int ty, bool txs) {
     const int it0 = (int)(it - 1);
     Txs<T> X = make_txs<T>(s);
     const bool mark = s->time_sweep > 0 && s->sweep_seq % s->time_sweep == 0;
     ++s->sweep_seq;
     if (fused) {
         Fused<T> F = make_fused<T>(s, ty);
+        TxSweep<T> XS{};
+        if (txs) {
+            XS = tx_sweep_of<T>(s, X);
+            s->txs_epoch = s->txs_epoch % 0x7ffffffe + 1;  // never 0 (the flags' initial value)
+            XS.epoch = s->txs_epoch;
+            XS.nbt = s->ntl[LIST_ALL];
+            if (!s->txs_pending) XS.nrb = 0;
+            XS.it_rec = it0 - 1;
+            XS.it0 = it0;
+        }
         sweep_mark(s, st, mark);
-        launch_fused<T, EXACT>(s, F, fused_cpl(s), st);
+        launch_fused<T, EXACT>(s, F, fused_cpl(s), st, txs ? &XS : nullptr);
         sweep_mark(s, st, mark);
         s->cur ^= 1;  // the new buffers become current
         Solve<T> S = make_solve<T>(s);
         const bool one = s->ntx && s->txc_ok && s->tx_launch;
-        if (one) {
+        if (txs) {
+            s->txs_pending = true;
+        } else if (one) {
             // every transceiver of the step in one launch (F still holds
             // old = 0 and new = 1 buffers)
-            TxChain C{};
-            C.list = s->txc_list;
-            C.hdr = s->txc_hdr;
-            C.rec = s->txc_list + s->txc_rec_off;
-            C.nchain = s->txc_nchain;
-            C.n_rec = s->txc_nrec;
-            C.nlp = (int)s->lev_p.size() - 1;
-            C.nlv = (int)s->lev_v.size() - 1;
+            const TxChain C = tx_chain_of(s);
             const int nblk = C.nchain + (C.n_rec + 7) / 8;
             if (nblk)
                 hipLaunchKernelGGL((k_tx_step<T, EXACT>), dim3(nblk), dim3(256), 0, st, F, X, C, it0, gf_hist(s));
@@ -1366,11 +1534,18 @@ static int run_impl(adf_solver* s, int64_t start, int64_t end) {
         return fail(ADF_EINVAL, "fused path not available for this grid at these cells per lane");
     const bool fused = s->path == 2 || (s->path == 0 && fused_supported(s));
     int ty = 2, kb = 16, kbg = 8;
+    bool txs = false;
     if (fused) {
         pick_tiles(s, &ty, &kb, &kbg);
         int rc = prepare_tiles(s, ty, kb, kbg);
         if (rc) return rc;
+        if (s->tx_launch == 2 && s->ntx && s->txc_ok && s->launch == ADF_SWEEP_ONE_LAUNCH &&
+            (s->waves ? s->waves : 1) == 1 && !s->gf_kind) {
+            if (s->txs_state == 0 && (rc = build_txs(s, ty))) return rc;
+            txs = s->txs_state == 1;
+        }
     }
+    s->txs_pending = false;
     s->sweep_used = 0;
     s->sweep_avg_ms = 0.0;
     s->sweep_n = 0;
@@ -1388,7 +1563,7 @@ static int run_impl(adf_solver* s, int64_t start, int64_t end) {
     }
     HIPCHK(hipEventRecord(s->ev0, st));
     for (int64_t it = start; it <= end; ++it) {
-        step_impl<T, EXACT>(s, it, st, fused, ty);
+        step_impl<T, EXACT>(s, it, st, fused, ty, txs);
         if (s->time_sweep && s->sweep_used >= 4096) {  // bound the event pool: flush
             HIPCHK(hipStreamSynchronize(st));
             for (int e = 0; e + 1 < s->sweep_used; e += 2) {
@@ -1401,10 +1576,25 @@ static int run_impl(adf_solver* s, int64_t start, int64_t end) {
             s->sweep_used = 0;
         }
     }
+    if (s->txs_pending) {
+        // the last step's records, from the current buffers
+        TxSweep<T> XS = tx_sweep_of<T>(s, make_txs<T>(s));
+        XS.it_rec = (int)(end - 1);
+        if (XS.nrb)
+            hipLaunchKernelGGL((k_tx_records<T>), dim3(XS.nrb), dim3(64), 0, st, make_fused<T>(s, ty), XS);
+        s->txs_pending = false;
+    }
+    // the wait flag comes back with the run's last event (no extra sync)
+    if (txs) HIPCHK(hipMemcpyAsync(s->txs_err_h, s->txs_err, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipGetLastError());
     if (s->gf_kind && s->gf_wb) s->gf_hist_valid = std::max(s->gf_hist_valid, end);
     HIPCHK(hipEventRecord(s->ev1, st));
     HIPCHK(wait_event(s->ev1));
+    if (txs && *s->txs_err_h) {
+        *s->txs_err_h = 0;
+        HIPCHK(hipMemset(s->txs_err, 0, 4));
+        return fail(ADF_EDEVICE, "transceiver step: a chain block's wait for its tiles ran out (results invalid)");
+    }
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
     const int64_t nsteps = end - start + 1;

@@ -620,9 +620,28 @@ __device__ __forceinline__ void sweep_tile(const Fused<T>& F, uint2 e) {
 
 // One solve's tiles of one list (entries as xcd_tile describes); W waves
 // per workgroup take W consecutive tiles.  POL: cache policy of the
-// stores (0 or ST_NT).
+// stores (0 or ST_NT).  With S.nbt > 0 (one wave per workgroup) the step's
+// transceiver work rides in the launch (TxSweep): S.nrb record blocks
+// first, then S.nbt blocks of one tile each, then the chain blocks.
 template <typename T, bool EXACT, int CPL, bool PF, int POL, bool RAG, bool INTONLY = false>
-__global__ __launch_bounds__(256) void k_sweep(Fused<T> F, const uint2* __restrict__ tiles, int ntiles) {
+__global__ __launch_bounds__(256) void k_sweep(Fused<T> F, const uint2* __restrict__ tiles, int ntiles,
+                                               TxSweep<T> S) {
+    if (S.nbt > 0) {
+        const int b = (int)blockIdx.x - S.nrb;
+        if (b < 0) {
+            tx_sweep_records<T>(F, S, b + S.nrb);
+            return;
+        }
+        if (b >= S.nbt) {
+            tx_sweep_chain<T, EXACT>(F, S, b - S.nbt);
+            return;
+        }
+        const int t = xcd_tile(b, S.nbt);
+        const uint2 e = tiles[t];
+        sweep_tile<T, EXACT, CPL, PF, POL, RAG, INTONLY>(F, e);
+        if (e.x & TILE_SIGNAL) tx_tile_signal<T>(S, t);
+        return;
+    }
     const int W = (int)(blockDim.x >> 6);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x) * W + w;

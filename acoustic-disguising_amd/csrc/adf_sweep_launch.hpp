@@ -15,35 +15,42 @@ namespace adf {
 // at 4 cells per lane only with 1-row interior tiles (sweep_ty_int).
 
 // The tiles of list LIST on stream st.  RAG: rows not a multiple of CPL
-// long.
+// long.  X (LIST_ALL only): the step's transceiver work rides in the launch
+// (TxSweep), over the solve's signal-marked tile order.
 template <typename T, bool EXACT, int CPL, bool RAG, int LIST>
-static void launch_list(adf_solver* s, const Fused<T>& F, hipStream_t st) {
+static void launch_list(adf_solver* s, const Fused<T>& F, hipStream_t st, const TxSweep<T>* X = nullptr) {
     const int n = s->ntl[LIST];
     if (!n) return;
     const int W = s->waves ? s->waves : 1;
-    const dim3 g((unsigned)((n + W - 1) / W));
+    dim3 g((unsigned)((n + W - 1) / W));
     const uint2* tl = (const uint2*)s->tiles[LIST];
+    TxSweep<T> S{};
+    if (X) {
+        S = *X;
+        tl = s->txs_tiles;
+        g = dim3((unsigned)(S.nbt + S.nrb + S.C.nchain));
+    }
     // the interior list holds only unmasked interior tiles: its launch
     // takes the kernel that compiles their routine alone
     constexpr bool IO = LIST == LIST_INT;
     if (s->pipeline && s->store_nt > 0) {  // automatic: ordinary stores for a single solve
-        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, ST_NT, RAG, IO>), g, dim3(64 * W), 0, st, F, tl, n);
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, ST_NT, RAG, IO>), g, dim3(64 * W), 0, st, F, tl, n, S);
         return;
     }
     if (s->pipeline) {
-        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, 0, RAG, IO>), g, dim3(64 * W), 0, st, F, tl, n);
+        hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, true, 0, RAG, IO>), g, dim3(64 * W), 0, st, F, tl, n, S);
         return;
     }
-    hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, false, 0, RAG>), g, dim3(64 * W), 0, st, F, tl, n);
+    hipLaunchKernelGGL((k_sweep<T, EXACT, CPL, false, 0, RAG>), g, dim3(64 * W), 0, st, F, tl, n, S);
 }
 
 // One launch over all tiles, or side and interior tiles as two launches
 // (on two streams of equal priority at the same time through fork/join
 // events, or one after the other).
 template <typename T, bool EXACT, int CPL, bool RAG>
-static void launch_fused_cpl(adf_solver* s, const Fused<T>& F, hipStream_t st) {
+static void launch_fused_cpl(adf_solver* s, const Fused<T>& F, hipStream_t st, const TxSweep<T>* X) {
     if (s->launch == ADF_SWEEP_ONE_LAUNCH) {
-        launch_list<T, EXACT, CPL, RAG, LIST_ALL>(s, F, st);
+        launch_list<T, EXACT, CPL, RAG, LIST_ALL>(s, F, st, X);
         return;
     }
     hipStream_t side = s->ctx->side;
@@ -62,13 +69,13 @@ static void launch_fused_cpl(adf_solver* s, const Fused<T>& F, hipStream_t st) {
 
 // Rows of one cell per lane are never ragged.
 template <typename T, bool EXACT>
-void launch_fused(adf_solver* s, const Fused<T>& F, int cpl, hipStream_t st) {
+void launch_fused(adf_solver* s, const Fused<T>& F, int cpl, hipStream_t st, const TxSweep<T>* X) {
     const bool rag = F.nx % cpl != 0;
-    if (cpl == 4 && rag) launch_fused_cpl<T, EXACT, 4, true>(s, F, st);
-    else if (cpl == 4) launch_fused_cpl<T, EXACT, 4, false>(s, F, st);
-    else if (cpl == 2 && rag) launch_fused_cpl<T, EXACT, 2, true>(s, F, st);
-    else if (cpl == 2) launch_fused_cpl<T, EXACT, 2, false>(s, F, st);
-    else launch_fused_cpl<T, EXACT, 1, false>(s, F, st);
+    if (cpl == 4 && rag) launch_fused_cpl<T, EXACT, 4, true>(s, F, st, X);
+    else if (cpl == 4) launch_fused_cpl<T, EXACT, 4, false>(s, F, st, X);
+    else if (cpl == 2 && rag) launch_fused_cpl<T, EXACT, 2, true>(s, F, st, X);
+    else if (cpl == 2) launch_fused_cpl<T, EXACT, 2, false>(s, F, st, X);
+    else launch_fused_cpl<T, EXACT, 1, false>(s, F, st, X);
 }
 
 

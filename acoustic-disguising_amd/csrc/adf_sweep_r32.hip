@@ -2,6 +2,6 @@
 #include "adf_sweep_launch.hpp"
 
 namespace adf {
-template void launch_fused<float, true>(adf_solver*, const Fused<float>&, int, hipStream_t);
+template void launch_fused<float, true>(adf_solver*, const Fused<float>&, int, hipStream_t, const TxSweep<float>*);
 template void launch_fused_batch<float, true>(adf_batch*, hipStream_t);
 }  // namespace adf

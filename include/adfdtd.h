@@ -232,14 +232,20 @@ int adf_solver_get_pressure_slice(adf_solver* s, int axis, int64_t index, void* 
  *                        takes each member's setting.  Results are
  *                        identical. */
 #define ADF_OPT_STORE_NT 9
-/*   ADF_OPT_TX_LAUNCH    1 (default): on the fused path, the transceivers
- *                        of a step run in one launch when the ones that
- *                        inject or read injected cells are few (at most 8
- *                        levels and 512 entries per phase, 4096 fix-up
- *                        cells): one workgroup runs those in order, the
- *                        others record; 0: one launch per transceiver level
- *                        and one for the velocity fix-up.  Results are
- *                        identical. */
+/*   ADF_OPT_TX_LAUNCH    1: on the fused path, the transceivers of a step
+ *                        run in one launch when the ones that inject or
+ *                        read injected cells are few (at most 8 levels and
+ *                        512 entries per phase, 4096 fix-up cells): one
+ *                        workgroup runs those in order, the others record;
+ *                        2 (default): as 1, but inside the sweep launch
+ *                        (one launch, one wave per workgroup, no GF map, at
+ *                        most 64 tiles holding cells those transceivers
+ *                        touch; otherwise as 1): their workgroups wait for
+ *                        the tiles that store those cells, and the others'
+ *                        records are taken by the next step's sweep launch
+ *                        (the last step's when the run returns); 0: one
+ *                        launch per transceiver level and one for the
+ *                        velocity fix-up.  Results are identical. */
 #define ADF_OPT_TX_LAUNCH 10
 /*   ADF_OPT_CELLS_PER_LANE 0 (default): the fused kernel's lanes hold 1
  *                        cell up to nx = 64, 2 up to 128, 4 above; or 1, 2

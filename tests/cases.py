@@ -74,7 +74,7 @@ def oracle_solver(O, d, txs, cpml, mask, mode, field=None):
 
 
 def gpu_run(d, txs, cpml, mask, dtype, precision, field=None, gf=None, path="auto", tile_y=0, block_z=0, waves=0,
-            pipeline=1, launch=0, store_nt=-1, tx_launch=1, cells_per_lane=0):
+            pipeline=1, launch=0, store_nt=-1, tx_launch=2, cells_per_lane=0):
     g = A.transceivers_to_grid(d, txs, dtype)
     f = np.zeros((4, d.nz, d.ny, d.nx), dtype) if field is None else np.array(field, dtype)
     cp = A.make_cpml(d, cpml.n, cpml.rcoef, cpml.fc, dtype)

@@ -22,10 +22,12 @@ TOL = {"f64": 1e-12, "ref": 1e-6, "fast": 1e-5}
 # with several tile shapes, waves per workgroup, the pipelined and the
 # plane-at-a-time routines, and the three launch forms) must give the same
 # results, with ordinary and non-temporal stores, and with the transceivers
-# of a step in one launch or in one launch per level.
+# of a step in one launch per level, in one launch, or in the sweep launch
+# (records one step later).
 # (path, tile_y, block_z, waves, pipeline, launch, store_nt, tx_launch)
 PATHS = [("two-pass", 0, 0, 0, 0, 0, 0, 1), ("fused", 0, 3, 1, 0, 0, 0, 0), ("fused", 0, 0, 2, 1, 1, 0, 1),
-         ("fused", 0, 5, 4, 1, 0, 1, 1), ("fused", 0, 2, 1, 1, 2, 0, 0)]
+         ("fused", 0, 5, 4, 1, 0, 1, 1), ("fused", 0, 2, 1, 1, 2, 0, 0), ("fused", 0, 4, 1, 1, 0, 1, 2),
+         ("fused", 0, 3, 1, 0, 0, 0, 2)]
 
 
 @pytest.fixture(params=PATHS, ids=lambda p: f"{p[0]}-ty{p[1]}-kb{p[2]}-w{p[3]}-pf{p[4]}-l{p[5]}-nt{p[6]}-tx{p[7]}")
@@ -290,12 +292,53 @@ def test_injection_surface_groups_one_launch_equals_levels():
     mask = A.build_update_mask(d, "sphere", [4 * d.dx])
     ref = oracle_solver(O, d, txs, cpml, mask, "f64")
     ref.run()
-    runs = [gpu_run(d, txs, cpml, mask, np.float64, "fast", path="fused", tx_launch=t) for t in (1, 0)]
-    (f1, g1, _), (f0, g0, _) = runs
-    assert np.array_equal(f1, f0)
-    assert np.array_equal(g1.rec, g0.rec) and np.array_equal(g1.vn_rec, g0.vn_rec)
-    assert np.array_equal(g1.src, g0.src)
+    runs = [gpu_run(d, txs, cpml, mask, np.float64, "fast", path="fused", tx_launch=t) for t in (1, 0, 2)]
+    (f1, g1, _), (f0, g0, _), (f2, g2, _) = runs
+    for f, g in ((f0, g0), (f2, g2)):
+        assert np.array_equal(f1, f)
+        assert np.array_equal(g1.rec, g.rec) and np.array_equal(g1.vn_rec, g.vn_rec)
+        assert np.array_equal(g1.src, g.src)
     errs = [rel_l2(g1.rec[c], ref.tx["rec"][c]) for c in range(4)]
     errs += [rel_l2(g1.vn_rec, ref.tx["vn_rec"]), rel_l2(g1.src, ref.tx["src"]), rel_l2(f1, ref.field)]
     print("f64 vs oracle", [float(f"{e:.3g}") for e in errs], flush=True)
     assert max(errs) <= TOL["f64"], errs
+
+
+def test_tx_in_sweep_split_runs_and_trace_reads():
+    # Transceivers carried by the sweep launch take each step's records one
+    # step later (in the next sweep launch, or in a closing launch at the
+    # end of a run).  Runs split at any step, a trace read between runs and
+    # a change of the transceiver launch form between runs must leave every
+    # trace and the field equal, bit for bit, to one run with the one-launch
+    # transceiver step.
+    from cases import domain_cells
+    d = domain_cells(36, 0.00686, 40)
+    rng = np.random.default_rng(12)
+    pts, nrm = A.fibonacci_sphere(40, 10 * d.dx)
+    rp, rn = A.fibonacci_sphere(30, 14 * d.dx)
+    txs = [A.make_transceiver(d, pts[i], [1.0, 0.0, 0.0, 0.0], rng.standard_normal(d.nt) * 1e-3) for i in range(40)]
+    txs += [A.make_transceiver(d, pts[i], np.r_[0.0, nrm[i]], rng.standard_normal(d.nt) * 1e-6) for i in range(40)]
+    txs += [A.make_transceiver(d, rp[i], np.r_[0.0, rn[i]], np.zeros(d.nt)) for i in range(30)]
+    mask = A.build_update_mask(d, "sphere", [4 * d.dx])
+
+    def run(parts):
+        cp = A.make_cpml(d, 5, 1e-4, 2000.0, np.float64)
+        s = A.Solver(d, A.dtype_code(np.float64, "fast"), cp)
+        s.set_mask(mask)
+        s.set_transceivers(A.transceivers_to_grid(d, txs, np.float64))
+        seen = []
+        for a, b, tx in parts:
+            s.set_path("fused", tx_launch=tx)
+            s.run(a, b)
+            seen.append(s.get_trace(0)[:, :b].copy())
+        return s.get_field(), [s.get_trace(w) for w in range(10)], seen
+
+    f1, t1, _ = run([(1, d.nt, 1)])
+    f2, t2, s2 = run([(1, 7, 2), (8, 8, 2), (9, 25, 1), (26, d.nt, 2)])
+    f3, t3, _ = run([(1, d.nt, 2)])
+    for f, t in ((f2, t2), (f3, t3)):
+        assert np.array_equal(f1, f)
+        for a, b in zip(t1, t):
+            assert np.array_equal(a, b)
+    for part, b in zip(s2, (7, 8, 25, d.nt)):  # every record of a run is there when it returns
+        assert np.array_equal(part, t1[0][:, :b])
