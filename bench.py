@@ -756,7 +756,7 @@ def main():
         "hbm_frac_step": round(b32 / (f32["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_sweep (field-update sweep of one step: one launch over interior and side tiles)",
+                     "kernel": "k_sweep (field-update sweep of one step: one launch over interior and side tiles, carrying the step's transceiver work)",
                      "algorithmic_bytes_per_launch": b32,
                      "avg_launch_ms": round(f32["sweep_ms"], 5), "launches": f32["sweep_launches"],
                      "timed_every": sweep_sample_period(args.steps),
