@@ -1346,9 +1346,14 @@ static int build_txs(adf_solver* s, int ty) {
     };
     std::vector<std::vector<int>> wl((size_t)nb);
     std::vector<uint8_t> sig((size_t)n, 0);
+    bool bad = false;  // a cell outside the grid (not expected): keep k_tx_step
     for (int b = 0; b < nb; ++b) {
         std::vector<int>& w = wl[b];
         auto add = [&](int64_t c) {
+            if (c < 0 || c >= s->n3) {
+                bad = true;
+                return;
+            }
             const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
             w.push_back(owner(i, j, k));
             if (j == 0 && ny > 1) w.push_back(owner(i, 1, k));
@@ -1383,7 +1388,7 @@ static int build_txs(adf_solver* s, int ty) {
     }
     // a signal tile's agent-scope release writes back its XCD's L2: worth it
     // for the few tiles of a point source's chain, not for a surface's
-    if (std::count(sig.begin(), sig.end(), (uint8_t)1) > TXS_MAX_SIGNAL) return ADF_OK;
+    if (bad || std::count(sig.begin(), sig.end(), (uint8_t)1) > TXS_MAX_SIGNAL) return ADF_OK;
     // signal tiles first within each XCD chunk (xcd_tile's chunks of n)
     std::vector<int> pos((size_t)n);
     std::vector<uint2> out;
