@@ -8,7 +8,9 @@
 // entries from an atomic counter, general (boundary) tiles first.  The
 // transceiver levels, the velocity fixups and the rigid mirrors of all
 // members run as one launch each, then k_advance moves every solve to the
-// next step.
+// next step.  When every member's transceivers fit the one-launch step,
+// they all run in one launch instead (k_tx_step_batch), which also
+// advances the members unless some need the separate mirror launch.
 #pragma once
 #include "adf_sweep.hpp"
 
@@ -177,6 +179,40 @@ __global__ void k_mirror_batch(const TxDev<T>* __restrict__ td, const int* __res
         }
         if (t < nx * ny) Vz[t] = -Vz[t + nx * ny];
     }
+}
+
+// Every member's transceiver work of one batched step in one launch: block b
+// runs block map[b].y of member map[b].x's one-launch transceiver step
+// (k_tx_step: its chain blocks, then its record blocks) on the member's
+// current buffers and step.  With adv, the last block to finish moves every
+// member to the next step as k_advance does: each block has read its
+// member's step and buffer parity (and used them) before it counts itself
+// finished, and the next kernels see the new values at the kernel
+// boundary, so the count needs no fence (an agent-scope fence per block
+// would write back its XCD's L2 each time).
+template <typename T, bool EXACT>
+__global__ __launch_bounds__(256) void k_tx_step_batch(const Fused<T>* __restrict__ fs,
+                                                       const TxDev<T>* __restrict__ td,
+                                                       const TxChain* __restrict__ chains,
+                                                       const int2* __restrict__ map, int* it0s, int* cur,
+                                                       unsigned* done, int nmem, int adv) {
+    const int2 m = map[blockIdx.x];
+    const int s = m.x;
+    const GfHist H{};
+    tx_step_block<T, EXACT>(fs[2 * s + cur[s]], td[s].X, chains[s], it0s[s], H, m.y);
+    if (!adv) return;
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(done, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    // the members in parallel (one thread walking them would wait a memory
+    // round trip per member: the two arrays may alias)
+    for (int t = (int)threadIdx.x; t < nmem; t += (int)blockDim.x) {
+        it0s[t] += 1;
+        cur[t] ^= 1;
+    }
+    if (threadIdx.x == 0) atomicExch(done, 0u);
 }
 
 static __global__ void k_advance(int* it0s, int* cur, int n) {

@@ -728,16 +728,17 @@ struct TxChain {
 // record-only transceivers each (32 lanes per transceiver): their records
 // read cells nothing in this launch writes, so they run beside the chain.
 template <typename T, bool EXACT>
-__global__ __launch_bounds__(256) void k_tx_step(Fused<T> F, Txs<T> X, TxChain C, int it0, GfHist H) {
+__device__ __forceinline__ void tx_step_block(const Fused<T>& F, const Txs<T>& X, const TxChain& C, int it0,
+                                              const GfHist& H, int blk) {
     const int tid = (int)threadIdx.x, lw = tid & 63;
     T* P = F.P1();
     T* Vx = F.V1(0);
     T* Vy = F.V1(1);
     T* Vz = F.V1(2);
-    if ((int)blockIdx.x < C.nchain) {
+    if (blk < C.nchain) {
         // the block's offsets: one vector load, then lane reads (a scalar
         // load per offset would wait a memory round trip each)
-        const int hv = lw < TXC_HDR ? C.hdr[(size_t)blockIdx.x * TXC_HDR + lw] : 0;
+        const int hv = lw < TXC_HDR ? C.hdr[(size_t)blk * TXC_HDR + lw] : 0;
         auto h = [&](int i) { return __builtin_amdgcn_readlane(hv, i); };
         // Pressure entries fill the block from thread 0, velocity entries
         // from thread 64 and fix-up cells from thread 128 (rotated), so with
@@ -788,9 +789,14 @@ __global__ __launch_bounds__(256) void k_tx_step(Fused<T> F, Txs<T> X, TxChain C
         }
         return;
     }
-    const int g = ((int)blockIdx.x - C.nchain) * 8 + (tid >> 5);
+    const int g = (blk - C.nchain) * 8 + (tid >> 5);
     const bool any = g < C.n_rec;
     tx_record_entry<T>(P, Vx, Vy, Vz, F.nx, F.ny, F.nz, X, any ? (int)C.rec[g] : 0, any, lw, it0, H);
+}
+
+template <typename T, bool EXACT>
+__global__ __launch_bounds__(256) void k_tx_step(Fused<T> F, Txs<T> X, TxChain C, int it0, GfHist H) {
+    tx_step_block<T, EXACT>(F, X, C, it0, H, (int)blockIdx.x);
 }
 
 // Transceiver work carried by a single solve's sweep launch

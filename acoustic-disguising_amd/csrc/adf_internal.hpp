@@ -211,6 +211,13 @@ struct adf_batch {
     int n_fix = 0;
     int* mirror = nullptr;
     int n_mirror = 0;
+    // one launch for every member's transceivers (k_tx_step_batch): each
+    // member's chain lists, per block (member, block of its step), and the
+    // launch's finished-block count; n_txmap 0: the per-level launches
+    void* chains = nullptr;
+    int2* txmap = nullptr;
+    int n_txmap = 0;
+    unsigned* txdone = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int gexec_steps = 0;

@@ -1711,7 +1711,11 @@ static void batch_drop_tables(adf_batch* b) {
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
     b->gexec_steps = 0;
     dfree(b->fs); dfree(b->td); dfree(b->it0); dfree(b->cur); dfree(b->txp); dfree(b->txv); dfree(b->fix);
-    dfree(b->mirror);
+    dfree(b->mirror); dfree(b->chains); dfree(b->txmap); dfree(b->txdone);
+    b->chains = nullptr;
+    b->txmap = nullptr;
+    b->txdone = nullptr;
+    b->n_txmap = 0;
     b->fs = b->td = nullptr;
     b->it0 = b->cur = b->mirror = nullptr;
     b->txp = b->txv = nullptr;
@@ -1907,6 +1911,33 @@ static int batch_build(adf_batch* b) {
         if ((rc = dmalloc((void**)&b->mirror, mir.size() * 4, "batch mirrors"))) return rc;
         HIPCHK(hipMemcpy(b->mirror, mir.data(), mir.size() * 4, hipMemcpyHostToDevice));
     }
+    // every member's transceivers in one launch when each fits the
+    // one-launch step (ADF_OPT_TX_LAUNCH not 0, no GF map)
+    bool one = true;
+    std::vector<TxChain> ch((size_t)n);
+    std::vector<int2> map;
+    for (int m = 0; m < n && one; ++m) {
+        adf_solver* s = b->members[m];
+        ch[m] = TxChain{};
+        if (!s->ntx) continue;
+        if (!s->txc_ok || !s->tx_launch || s->gf_kind) {
+            one = false;
+            break;
+        }
+        ch[m] = tx_chain_of(s);
+        const int nblk = ch[m].nchain + (ch[m].n_rec + 7) / 8;
+        for (int k = 0; k < nblk; ++k) map.push_back(make_int2(m, k));
+    }
+    if (one && !map.empty()) {
+        if ((rc = dmalloc(&b->chains, ch.size() * sizeof(TxChain), "batch transceiver chains")) ||
+            (rc = dmalloc((void**)&b->txmap, map.size() * sizeof(int2), "batch transceiver chains")) ||
+            (rc = dmalloc((void**)&b->txdone, sizeof(unsigned), "batch transceiver chains")))
+            return rc;
+        HIPCHK(hipMemcpy(b->chains, ch.data(), ch.size() * sizeof(TxChain), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->txmap, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(b->txdone, 0, sizeof(unsigned)));
+        b->n_txmap = (int)map.size();
+    }
     b->dirty = false;
     return ADF_OK;
 }
@@ -1918,6 +1949,16 @@ template <typename T, bool EXACT>
 static void batch_step(adf_batch* b, hipStream_t st) {
     launch_fused_batch<T, EXACT>(b, st);
     const TxDev<T>* td = (const TxDev<T>*)b->td;
+    const int nm = (int)b->members.size();
+    if (b->n_txmap) {
+        const int adv = b->n_mirror == 0;
+        hipLaunchKernelGGL((k_tx_step_batch<T, EXACT>), dim3(b->n_txmap), dim3(256), 0, st, (const Fused<T>*)b->fs,
+                           td, (const TxChain*)b->chains, b->txmap, b->it0, b->cur, b->txdone, nm, adv);
+        if (adv) return;
+        hipLaunchKernelGGL((k_mirror_batch<T>), dim3(64, b->n_mirror), dim3(256), 0, st, td, b->cur, b->mirror);
+        hipLaunchKernelGGL(k_advance, dim3((nm + 255) / 256), dim3(256), 0, st, b->it0, b->cur, nm);
+        return;
+    }
     for (size_t L = 0; L + 1 < b->txp_lev.size(); ++L) {
         const int n = b->txp_lev[L + 1] - b->txp_lev[L];
         if (n > 0)

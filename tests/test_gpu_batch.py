@@ -84,6 +84,29 @@ def test_batch_equals_solo(dtype, prec, persistent, graph):
         assert_same(a, c, f"member {i}")
 
 
+@pytest.mark.parametrize("tx", [0, 1])
+def test_batch_transceiver_launch_forms(tx):
+    """The members' transceivers as the batch's per-level launches (the
+    members' ADF_OPT_TX_LAUNCH 0) or as one launch for every member, which
+    then also moves the members to the next step (no member here needs the
+    separate mirror pass), across graph replays: each member equals its solo
+    run bit for bit."""
+    cs = [c for i, c in enumerate(cases()) if i != 2]
+    solo = [make_solver(c, np.float64) for c in cs]
+    for s in solo:
+        s.run(1, s.dom.nt)
+    mem = [make_solver(c, np.float64) for c in cs]
+    b = A.Batch(graph_steps=16)
+    for s in mem:
+        s.set_option(L.ADF_OPT_TX_LAUNCH, tx)
+        b.add(s)
+    b.run(33)
+    b.run(cs[0][0].nt - 33)
+    for i, (a, c) in enumerate(zip(mem, solo)):
+        assert a.next_step() == c.dom.nt + 1
+        assert_same(a, c, f"member {i}")
+
+
 def test_batch_f64_matches_oracle():
     cs = cases()
     b = A.Batch()
