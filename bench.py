@@ -702,8 +702,9 @@ def parse_args(argv=None):
 def main():
     args = parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
-              "independent solves shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (--gpus {args.gpus} requested); the library runs each "
+              "solve and batch on one GPU, and independent solves would shard across GPUs without "
+              "communication", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         if args.probe_kind >= 0:
