@@ -8,8 +8,8 @@ recording -> injection bridge e, the Kirchhoff-Helmholtz quadrature
 weights 4 pi r^2 / N of both surfaces, the impedance conversion z0 and the
 one-way (pin) or split (pos/neg) combinations.
 
-The reference reads its GF files with HDF5.jl; no HDF5 library exists in
-this build, so the loaders take the file's datasets and attributes as a
+The reference reads its GF files with HDF5.jl; this package has no HDF5
+reader (h5py is not a dependency), so the loaders take the file's datasets and attributes as a
 mapping (`read_gf_file` reads them from an .npz container with the same
 names: datasets p_p, p_v, v_p, v_v, t; attributes dom_dt, dom_dx, backend,
 units_convention, empirical_scale).
@@ -80,8 +80,8 @@ def gf_to_raw_fdtd(loaded: Mapping, C_p: float, C_v: float) -> dict:
 
 def read_gf_file(path: str) -> dict:
     """The datasets and attributes of a GF file as one dict.  .npz
-    containers (numpy, no pickles) are read here; HDF5 files need h5py,
-    which this build does not have."""
+    containers (numpy, no pickles) are read here; HDF5 files would need
+    h5py, which this package does not depend on."""
     if path.endswith(".npz"):
         with np.load(path, allow_pickle=False) as z:
             out = {k: z[k] for k in z.files}

@@ -603,7 +603,7 @@ def kh_leg(which="p_v"):
             "tflops_window": round(tflops, 2), "frac_fp32_valu_peak": round(tflops / 157.3, 4),
             "frac_fp32_fma_unpacked_peak": round(tflops / 78.6, 4),
             "dense_equivalent_tmacs_per_s": round(dense / (ms * 1e-3) / 1e12, 1),
-            "note": "157.3 TFLOP/s counts packed FP32 FMAs (two per lane per cycle); this build compiles without "
+            "note": "157.3 TFLOP/s counts packed FP32 FMAs (two per lane per cycle); the library is compiled without "
                     "them, which caps FP32 FMAs at 78.6 TFLOP/s"}
 
 

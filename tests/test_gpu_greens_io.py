@@ -1,6 +1,6 @@
 """The GF loaders on the device: load_gf_ref with the device's cubic
 resample equals the host restatement within float32 rounding, and the
-paper's chain runs end to end on this build: impulsive_gfs on the device,
+paper's chain runs end to end on the device: impulsive_gfs on the GPU,
 its GFs to physical units (gf_unit_factors, gf_to_physical, as the
 impulsive script saves them), an .npz container, load_gf_ref, and
 build_hologram(RefMode) with that map, whose traces equal the same map run
