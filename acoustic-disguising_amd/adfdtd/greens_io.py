@@ -9,8 +9,8 @@ weights 4 pi r^2 / N of both surfaces, the impedance conversion z0 and the
 one-way (pin) or split (pos/neg) combinations.
 
 The reference reads its GF files with HDF5.jl; this package has no HDF5
-reader (h5py is not a dependency), so the loaders take the file's datasets and attributes as a
-mapping (`read_gf_file` reads them from an .npz container with the same
+reader (h5py is not a dependency), so the loaders take the file's datasets
+and attributes as a mapping (`read_gf_file` reads them from an .npz container with the same
 names: datasets p_p, p_v, v_p, v_v, t; attributes dom_dt, dom_dx, backend,
 units_convention, empirical_scale).
 
