@@ -10,9 +10,9 @@ one-way (pin) or split (pos/neg) combinations.
 
 The reference reads its GF files with HDF5.jl; this package has no HDF5
 reader (h5py is not a dependency), so the loaders take the file's datasets
-and attributes as a mapping (`read_gf_file` reads them from an .npz container with the same
-names: datasets p_p, p_v, v_p, v_v, t; attributes dom_dt, dom_dx, backend,
-units_convention, empirical_scale).
+and attributes as a mapping (`read_gf_file` reads them from an .npz
+container with the same names: datasets p_p, p_v, v_p, v_v, t; attributes
+dom_dt, dom_dx, backend, units_convention, empirical_scale).
 
 Layouts: the reference's (nt, n_rec, n_src) GF arrays are numpy
 (n_src, n_rec, nt) arrays here, as everywhere in the package; the GF maps
